@@ -1,0 +1,233 @@
+"""Benchmark of the MI355X permanent engine.
+
+Headline workload (one "step" = one evaluation of the whole batch): 4096
+permanents of uneven size, n ~ U{8..24}, each a row/column selection with
+multiplicities 1..3 of a 48-mode Haar unitary, complex128, double-double
+accumulation, evaluated by one persistent work-stealing launch per
+column-width bucket replayed from a hipGraph.  Inputs are resident in device
+memory before the timed region (planning and upload happen once, outside).
+
+Printed as ONE JSON line (rank 0):
+  value           permanents/s of the batched engine (whole job, 1 GPU)
+  plain           the same batch through the plain FP64 kernel, one launch per
+                  permanent (speedup_vs_plain = value / plain rate)
+  n32             one n = 32 permanent (2^31 Gray terms): time, % of FP64 peak
+  rel_err_max     vs the double-double oracle values committed with the inputs
+  roofline        the sweep kernel: algorithmic FLOP / kernel time vs 78.6 TFLOP/s
+  cpu_baseline    the reference's own C++ (oracle/_ref) on the host cores, a
+                  bounded sample of the same batch
+
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+"""
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (and matrix) peak, AMD specification
+METRIC = "permanents/s batched n\u2208[8,24] and n=32 %FP64 peak, vs plain, rel.err\u22641e-10"
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def _load_batched():
+    from piquasso_amd import workloads as W
+
+    fx = np.load(os.path.join(GOLDEN, "batched_uneven_seed4.npz"))
+    mats, rows, cols = W.batched_uneven(seed=4, unitary=fx["unitary"])
+    dd = fx["dd"]
+    ref = (dd[:, 0] + dd[:, 1]) + 1j * (dd[:, 2] + dd[:, 3])
+    return mats, rows, cols, ref
+
+
+def _load_n32():
+    with open(os.path.join(GOLDEN, "extended_vectors.json")) as f:
+        cases = {c["id"]: c for c in json.load(f)["cases"]}
+    c = cases["large_single_n32"]
+    m = c["matrix"]
+    a = (np.asarray(m["re"]) + 1j * np.asarray(m["im"])).reshape(m["shape"])
+    dd = c["dd"]
+    return a, np.asarray(c["rows"]), np.asarray(c["cols"]), complex(dd[0] + dd[1], dd[2] + dd[3])
+
+
+# ------------------------------------------------------------ CPU leg --
+
+def cpu_baseline_child(seconds: float) -> None:
+    """Runs in a child process (before the parent touches the GPU): time the
+    reference's own permanent (oracle/_ref, built from its sources) on a
+    prefix of the batch until `seconds` of wall time are used."""
+    from oracle import oracle as O
+    from piquasso_amd import workloads as W
+
+    mats, rows, cols, _ = _load_batched()
+    kind = "reference"
+    try:
+        fn = O.ref_module().permanent
+    except O.OracleUnavailable:
+        kind = "port"
+        fn = lambda a, r, c: O.perm_port(a, r, c)  # noqa: E731
+    done = 0
+    flops = 0.0
+    t0 = time.perf_counter()
+    while done < len(mats) and time.perf_counter() - t0 < seconds:
+        fn(mats[done], rows[done], cols[done])
+        flops += W.glynn_flops(rows[done], cols[done])
+        done += 1
+    dt = time.perf_counter() - t0
+    print(json.dumps({"kind": kind, "items": done, "seconds": dt, "flops": flops,
+                      "threads": int(os.environ.get("OMP_THREAD_LIMIT", "0") or 0)}))
+
+
+def run_cpu_baseline(seconds: float):
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(16, cores or 1))
+    env = dict(os.environ)
+    # the reference asks for 4 x hardware_concurrency OpenMP threads per call
+    # (src/permanent.cpp:144-153); cap them at the host's CPU share
+    env["OMP_THREAD_LIMIT"] = str(threads)
+    env["OMP_NUM_THREADS"] = str(threads)
+    try:
+        out = subprocess.run(
+            [sys.executable, os.path.abspath(__file__), "--cpu-child", str(seconds)],
+            env=env, capture_output=True, text=True, timeout=seconds * 4 + 120,
+        )
+        rec = json.loads(out.stdout.strip().splitlines()[-1])
+    except Exception as e:  # the GPU numbers are still worth reporting
+        return {"value": None, "unit": "permanents/s", "cores": threads, "kind": "reference",
+                "sample": f"failed: {e}"}
+    rate = rec["items"] / rec["seconds"] if rec["seconds"] > 0 else None
+    return {
+        "value": rate,
+        "unit": "permanents/s",
+        "cores": threads,
+        "kind": rec["kind"],
+        "sample": (f"first {rec['items']} of the 4096 batched permanents "
+                   f"({rec['flops']:.3e} algorithmic FLOP) in {rec['seconds']:.1f} s, "
+                   f"OMP_THREAD_LIMIT={threads}"),
+        "gflops": rec["flops"] / rec["seconds"] / 1e9 if rec["seconds"] > 0 else None,
+    }
+
+
+# ------------------------------------------------------------ GPU leg --
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--cpu-child", type=float, default=None)
+    args = ap.parse_args()
+
+    if args.cpu_child is not None:
+        cpu_baseline_child(args.cpu_child)
+        return 0
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "here (every measurement is of the single-GPU engine)", file=sys.stderr)
+        return 2
+
+    cpu = None if args.skip_cpu else run_cpu_baseline(args.cpu_seconds)
+
+    import piquasso_amd as pa
+    from piquasso_amd import _capi, workloads as W
+
+    n_dev, cus, arch = _capi.device_info()
+    mats, rows, cols, ref = _load_batched()
+    flops_batch = sum(W.glynn_flops(r, c) for r, c in zip(rows, cols))
+
+    batch = pa.PermanentBatch(mats, rows, cols)
+    for _ in range(args.warmup):
+        batch.run("graph")
+    batch.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run("graph")
+    batch.sync()
+    wall = time.perf_counter() - t0
+    vals = batch.results()
+    rel = np.abs(vals - ref) / np.abs(ref)
+    step_ms = wall / args.steps * 1e3
+    value = len(mats) * args.steps / wall
+    graph_ms = batch.time("graph", reps=max(3, min(args.steps, 10)))
+    kernel_ms = batch.time("kernel", reps=max(3, min(args.steps, 10)))
+    plain_ms = batch.time("plain", reps=2)
+    stats = batch.stats()
+    kernel_avg = float(np.mean(kernel_ms))
+    achieved = flops_batch / (kernel_avg * 1e-3) / 1e12
+
+    # single large permanent, n = 32 (2^31 Gray terms)
+    a32, r32, c32, ref32 = _load_n32()
+    b32 = pa.PermanentBatch([a32], [r32], [c32], persistent=False)
+    b32.run("eager")
+    v32 = b32.results()[0]
+    t32 = b32.time("kernel", reps=3)
+    f32 = W.glynn_flops(r32, c32)
+    t32_ms = float(np.median(t32))
+    n32 = {
+        "ms": t32_ms,
+        "tflops": f32 / (t32_ms * 1e-3) / 1e12,
+        "frac_fp64_peak": f32 / (t32_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+        "rel_err": abs(v32 - ref32) / abs(ref32),
+        "flops": f32,
+    }
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "permanents/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded 48-mode Haar unitary committed with the oracle values)",
+        "config": {
+            "workload": "batched_uneven: 4096 permanents, n~U{8..24}, row/col multiplicities "
+                        "1..3 of a 48-mode Haar unitary, complex128, dd accumulation, "
+                        "persistent launch replayed from a hipGraph",
+            "items": len(mats),
+            "n_range": [8, 24],
+            "modes": 48,
+            "seed": 4,
+            "device": f"{arch} ({cus} CUs)",
+        },
+        "rel_err_max": float(rel.max()),
+        "graph_ms_median": float(np.median(graph_ms)),
+        "plain": {
+            "permanents_per_s": len(mats) / (float(np.median(plain_ms)) * 1e-3),
+            "ms": float(np.median(plain_ms)),
+            "what": "plain FP64 kernel, one launch + finalize per permanent, same device inputs",
+        },
+        "speedup_vs_plain": float(np.median(plain_ms)) / float(np.median(graph_ms)),
+        "n32": n32,
+        "roofline": {
+            "bound": "valu-fp64",
+            "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": None,
+            "kernel_ms": kernel_avg,
+            "flops_per_launch": flops_batch,
+            "units": stats["units"],
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

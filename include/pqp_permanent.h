@@ -1,0 +1,148 @@
+/*
+ * pqp_permanent.h — C ABI of the MI355X permanent engine (libpqp_permanent.so).
+ *
+ * The engine evaluates matrix permanents with row and column multiplicities,
+ *
+ *   perm(A; r, c) = permanent of the n x n matrix that repeats row i of A
+ *                   r_i times and column j c_j times,  n = sum r = sum c,
+ *
+ * by Glynn's formula with multiplicities over a reflected mixed-radix Gray
+ * code, the method of the reference's permanent_cpp<T>
+ * (src/permanent.cpp:49-264, src/n_aryGrayCodeCounter.hpp), on AMD gfx950
+ * GPUs.  Every sum and product runs on the GPU; the host only validates,
+ * compacts and tabulates.  There is no CPU fallback: without a usable HIP
+ * device every compute entry point returns PQP_ERR_DEVICE.
+ *
+ * Conventions
+ *  - Matrices are row-major, interleaved complex (re, im), leading dimension
+ *    `ld` in complex elements (ld >= m_c).  Host pointers; inputs are read
+ *    only for the duration of the call and never written.
+ *  - Multiplicities are int32, one per row (length m_r) and per column
+ *    (length m_c); zero entries are allowed.
+ *  - Return value: PQP_OK (0) or a negative PQP_ERR_* code; the message of
+ *    the last failure on the calling thread is pqp_last_error().  Nothing
+ *    throws across this boundary.
+ *  - Thread-safe: calls may come from several threads (they serialise on
+ *    the device they share).
+ */
+#ifndef PQP_PERMANENT_H
+#define PQP_PERMANENT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PQP_OK 0
+#define PQP_ERR_SUM_MISMATCH (-1) /* sum(rows) != sum(cols): the reference throws (src/permanent.cpp:100-104) */
+#define PQP_ERR_INVALID (-2)      /* bad shape, negative multiplicity, null pointer */
+#define PQP_ERR_RANGE (-3)        /* more than 64 columns / particles, or too many Gray terms */
+#define PQP_ERR_DEVICE (-4)       /* no HIP device, or a HIP runtime error */
+#define PQP_ERR_NOMEM (-5)        /* host or device allocation failed */
+#define PQP_ERR_NOKERNEL (-6)     /* no kernel variant for this request */
+
+/* Flags */
+#define PQP_FLAG_PLAIN 0x1u  /* plain FP64 accumulation and one launch per permanent (the baseline) */
+#define PQP_FLAG_FP32 0x2u   /* FP32 row sums and products, FP64 + double-double accumulation */
+#define PQP_FLAG_STATIC 0x4u /* batches: one block per work unit instead of the persistent queue */
+
+/* Library version string. */
+const char *pqp_version(void);
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char *pqp_last_error(void);
+
+/* Device 0 properties: returns PQP_OK and fills the outputs, or
+ * PQP_ERR_DEVICE when no HIP device is usable.  `name` may be null. */
+int pqp_device_info(int *n_devices, int *cu_count, char *name, int name_len);
+
+/*
+ * One permanent, complex128 in and out.
+ * Replaces permanent_cpp<double> (src/permanent.cpp:49-264) as reached from
+ * the pybind entry permanent(matrix, rows, cols)
+ * (piquasso/_math/permanent.cpp:26-41,76-78): same arguments, same result,
+ * empty or all-zero multiplicities give 1 (src/permanent.cpp:106-108), a
+ * row/column sum mismatch is PQP_ERR_SUM_MISMATCH.
+ *   out    : 2 doubles (re, im), the double-double result rounded to FP64
+ *   out_dd : 4 doubles (re_hi, re_lo, im_hi, im_lo) or null
+ *   flags  : 0, PQP_FLAG_PLAIN or PQP_FLAG_FP32
+ */
+int pqp_permanent_c128(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
+                       const int32_t *rows, const int32_t *cols, uint32_t flags,
+                       double *out, double *out_dd);
+
+/*
+ * One permanent, complex64 in and out.
+ * Replaces permanent_cpp<float> (src/permanent.cpp:266-267), the overload
+ * the pybind entry picks for complex64 input (piquasso/_math/permanent.cpp:73-75).
+ * Row sums and products run in FP32, the accumulation in FP64 +
+ * double-double (more accurate than the reference's all-FP32 sum).
+ *   out    : 2 floats (re, im)
+ *   out_dd : 4 doubles or null
+ */
+int pqp_permanent_c64(const float *matrix, int64_t m_r, int64_t m_c, int64_t ld,
+                      const int32_t *rows, const int32_t *cols, uint32_t flags, float *out,
+                      double *out_dd);
+
+/*
+ * Batches: many permanents of uneven size in one persistent, work-stealing
+ * launch (the reference has no batched entry; each item's value equals
+ * pqp_permanent_c128 on that item).  The batch owns its device buffers and
+ * stream; inputs are copied in at creation.
+ *   matrices : count pointers, complex128 (is_c64 = 0) or complex64 (1)
+ *   m_r, m_c, ld : count entries each
+ *   rows, cols   : count pointers to int32 multiplicities
+ *   target_inner : inner Gray states per lane (0 = the planner's cost model)
+ */
+typedef struct pqp_batch pqp_batch;
+
+int pqp_batch_create(int64_t count, const void *const *matrices, int is_c64, const int64_t *m_r,
+                     const int64_t *m_c, const int64_t *ld, const int32_t *const *rows,
+                     const int32_t *const *cols, uint32_t flags, int64_t target_inner,
+                     pqp_batch **out);
+
+/* Run modes for pqp_batch_run / pqp_batch_time */
+#define PQP_RUN_EAGER 0  /* persistent kernel(s) + finalize, launched on the batch stream */
+#define PQP_RUN_GRAPH 1  /* the same launches captured once in a hipGraph and replayed */
+#define PQP_RUN_PLAIN 2  /* plain FP64 kernel, one launch (+ finalize) per permanent */
+#define PQP_RUN_KERNEL 3 /* timing only: the sweep kernel(s) alone, finalize outside the events */
+
+/* Enqueue one evaluation of the whole batch (asynchronous). */
+int pqp_batch_run(pqp_batch *b, int mode);
+
+/* Wait for the batch stream. */
+int pqp_batch_sync(pqp_batch *b);
+
+/* Copy the results of the last run: out = 2 doubles per item, out_dd = 4
+ * doubles per item (may be null).  Synchronises first. */
+int pqp_batch_results(pqp_batch *b, double *out, double *out_dd);
+
+/* Time `reps` evaluations with HIP events recorded on the batch stream:
+ * ms[k] = elapsed milliseconds of evaluation k. */
+int pqp_batch_time(pqp_batch *b, int mode, int reps, float *ms);
+
+/* Planning statistics: algorithmic FLOP (sum of T * (2 m_c + 6 (n-1) + 4)),
+ * Gray terms, wave work units and device arena size.  Any pointer may be null. */
+int pqp_batch_stats(const pqp_batch *b, double *flops, int64_t *terms, int64_t *units,
+                    int64_t *arena_bytes);
+
+void pqp_batch_destroy(pqp_batch *b);
+
+/*
+ * Planning only (no device involved): the device arena the kernels would
+ * read for one permanent, for inspection by tests of the planner.  Layout:
+ * the ItemDesc record at offset 0, then the work units and tables at the
+ * offsets it names (piquasso_amd/csrc/pqp_types.hpp).  With buf == null only
+ * *needed_bytes is written.  No counterpart in the reference.
+ */
+int pqp_plan_image(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
+                   const int32_t *rows, const int32_t *cols, uint32_t flags,
+                   int64_t target_inner, int64_t min_units, void *buf, int64_t buf_bytes,
+                   int64_t *needed_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PQP_PERMANENT_H */

@@ -1,0 +1,74 @@
+/*
+ * ORACLE — test infrastructure, not product code.
+ *
+ * A CPU restatement of the reference's permanent path, used only by tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py as the checker
+ * (or the timed CPU baseline).  The product library under piquasso_amd/
+ * never links, loads or calls anything declared here.
+ *
+ * Two families:
+ *
+ *   pqo_perm_port_*   plain-precision restatement of permanent_cpp<T>
+ *                     (src/permanent.cpp:49-264): min-row prepend, Glynn /
+ *                     BBFG sum with multiplicities over the reflected n-ary
+ *                     Gray code (src/n_aryGrayCodeCounter.hpp:170-254), the
+ *                     same static chunking into `n_jobs` OpenMP jobs
+ *                     (src/permanent.cpp:144-165) and the same ordered
+ *                     reduction (:253-259).
+ *
+ *   pqo_perm_dd_*     the same sum evaluated in double-double arithmetic
+ *                     (TwoSum / FMA-TwoProd; Dekker 1971, Knuth TAOCP 4.2.2,
+ *                     Ogita-Rump-Oishi 2005): the extended-precision arbiter
+ *                     for relative-error claims.  pqo_grad_dd follows
+ *                     grad_perm (src/permanent.cpp:271-300): explicit minors
+ *                     r_i * c_j * perm(A; r - e_i, c - e_j).
+ *
+ * Conventions: `a` is row-major interleaved complex (re, im) with leading
+ * dimension m_c; multiplicities are int32; results are (re, im) pairs.
+ * Return 0 on success, -1 when sum(rows) != sum(cols) (the reference throws
+ * there, src/permanent.cpp:100-104), -2 on invalid arguments.
+ */
+#ifndef PQP_ORACLE_H
+#define PQP_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Plain FP64 restatement; n_jobs <= 0 means 4 * number of processors,
+ * as std::thread::hardware_concurrency() * 4 in src/permanent.cpp:145-147. */
+int pqo_perm_port_c128(const double *a, int m_r, int m_c, const int *rows,
+                       const int *cols, int n_jobs, double *out);
+
+/* Plain FP32 restatement (permanent_cpp<float>, src/permanent.cpp:266-267). */
+int pqo_perm_port_c64(const float *a, int m_r, int m_c, const int *rows,
+                      const int *cols, int n_jobs, float *out);
+
+/* Extended precision: result as double-double, out = {re_hi, re_lo, im_hi,
+ * im_lo}.  n_jobs <= 0 picks the processor count. */
+int pqo_perm_dd_c128(const double *a, int m_r, int m_c, const int *rows,
+                     const int *cols, int n_jobs, double *out);
+
+/* Gradient d perm / d A_ij (holomorphic), square or not: g is m_r x m_c
+ * interleaved complex, each entry r_i * c_j * perm(A; r - e_i, c - e_j)
+ * evaluated in double-double and rounded to double. */
+int pqo_grad_dd_c128(const double *a, int m_r, int m_c, const int *rows,
+                     const int *cols, int n_jobs, double *g);
+
+/* Mixed-radix reflected Gray code, restating n_aryGrayCodeCounter
+ * (src/n_aryGrayCodeCounter.hpp:170-254).  Writes the digit vectors of the
+ * Gray states with indices [offset, offset + count) into `codes`
+ * (count x n_digits, row-major).  Returns 0, or -2 on invalid input. */
+int pqo_gray_states(const int *limits, int n_digits, int64_t offset,
+                    int64_t count, int *codes);
+
+/* Number of processors OpenMP sees (used for the default chunking). */
+int pqo_num_procs(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PQP_ORACLE_H */

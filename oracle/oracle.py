@@ -1,0 +1,340 @@
+"""ORACLE — test infrastructure, not product code.
+
+Python face of the CPU oracle.  Only ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py`` import this module, and only as
+the checker (or the timed CPU baseline); the product package
+``piquasso_amd`` never imports it.
+
+Contents
+--------
+* ctypes bindings of ``liboracle.so`` (built from ``glynn_port.c`` and
+  ``glynn_dd.c`` by ``oracle/Makefile``):
+
+  - :func:`perm_port` / :func:`perm_port_c64` — plain-precision restatement of
+    ``permanent_cpp<T>`` (src/permanent.cpp:49-264), same Gray order and
+    chunking;
+  - :func:`perm_dd` — the same sum in double-double: the extended-precision
+    arbiter;
+  - :func:`grad_dd` — ``grad_perm`` (src/permanent.cpp:271-300) as explicit
+    minors in double-double;
+  - :func:`gray_states` — the reflected n-ary Gray counter
+    (src/n_aryGrayCodeCounter.hpp:170-254).
+
+* independent pure-Python / numpy formulas used to pin the above:
+
+  - :func:`assym_reduce` — multiplicity expansion
+    (piquasso/_math/linalg.py:98-117);
+  - :func:`perm_bruteforce` — sum over S_n, exact with ``fractions`` for
+    rational inputs;
+  - :func:`perm_ryser` — Ryser's inclusion-exclusion formula (Ryser 1963),
+    a different algorithm from Glynn's.
+
+* loaders for the reference's own code compiled by ``make ref`` into
+  ``oracle/_ref`` (absent unless that build ran): :func:`ref_module` (the
+  pybind ``permanent`` module, src: piquasso/_math/permanent.cpp:71-85) and
+  :func:`ref_lib` (ctypes shim around ``permanent_cpp`` / ``grad_perm``).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import importlib.util
+import itertools
+import os
+import sysconfig
+from fractions import Fraction
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+_lib = None
+
+
+class OracleUnavailable(RuntimeError):
+    pass
+
+
+def lib():
+    """Load ``liboracle.so`` (``make -C oracle`` builds it)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OracleUnavailable(
+                f"{LIB_PATH} is missing; run `make -C oracle` first"
+            )
+        L = ctypes.CDLL(LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        fp = ctypes.POINTER(ctypes.c_float)
+        ip = ctypes.POINTER(ctypes.c_int)
+        ci = ctypes.c_int
+        for name, argt in (
+            ("pqo_perm_port_c128", [dp, ci, ci, ip, ip, ci, dp]),
+            ("pqo_perm_port_c64", [fp, ci, ci, ip, ip, ci, fp]),
+            ("pqo_perm_dd_c128", [dp, ci, ci, ip, ip, ci, dp]),
+            ("pqo_grad_dd_c128", [dp, ci, ci, ip, ip, ci, dp]),
+            ("pqo_gray_states", [ip, ci, ctypes.c_int64, ctypes.c_int64, ip]),
+        ):
+            fn = getattr(L, name)
+            fn.argtypes = argt
+            fn.restype = ci
+        L.pqo_num_procs.restype = ci
+        L.pqo_num_procs.argtypes = []
+        _lib = L
+    return _lib
+
+
+def _prep(matrix, rows, cols, dtype=np.complex128):
+    a = np.ascontiguousarray(np.asarray(matrix, dtype=dtype))
+    if a.ndim != 2:
+        raise ValueError("matrix must be 2-D")
+    r = np.ascontiguousarray(np.asarray(rows, dtype=np.int32).reshape(-1))
+    c = np.ascontiguousarray(np.asarray(cols, dtype=np.int32).reshape(-1))
+    if r.shape[0] != a.shape[0] or c.shape[0] != a.shape[1]:
+        raise ValueError("rows/cols lengths must match the matrix shape")
+    return a, r, c
+
+
+def _ptr(x, ct):
+    return x.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def _check(status):
+    if status == -1:
+        raise RuntimeError("Number of input and output states should be equal")
+    if status != 0:
+        raise ValueError(f"oracle rejected the arguments (status {status})")
+
+
+def perm_port(matrix, rows, cols, n_jobs: int = 0) -> complex:
+    """Plain FP64 restatement of ``permanent_cpp<double>``."""
+    a, r, c = _prep(matrix, rows, cols)
+    out = np.zeros(2)
+    _check(
+        lib().pqo_perm_port_c128(
+            _ptr(a.view(np.float64), ctypes.c_double), a.shape[0], a.shape[1],
+            _ptr(r, ctypes.c_int), _ptr(c, ctypes.c_int), n_jobs,
+            _ptr(out, ctypes.c_double),
+        )
+    )
+    return complex(out[0], out[1])
+
+
+def perm_port_c64(matrix, rows, cols, n_jobs: int = 0) -> np.complex64:
+    """Plain FP32 restatement of ``permanent_cpp<float>``."""
+    a, r, c = _prep(matrix, rows, cols, dtype=np.complex64)
+    out = np.zeros(2, dtype=np.float32)
+    _check(
+        lib().pqo_perm_port_c64(
+            _ptr(a.view(np.float32), ctypes.c_float), a.shape[0], a.shape[1],
+            _ptr(r, ctypes.c_int), _ptr(c, ctypes.c_int), n_jobs,
+            _ptr(out, ctypes.c_float),
+        )
+    )
+    return np.complex64(complex(out[0], out[1]))
+
+
+def perm_dd_parts(matrix, rows, cols, n_jobs: int = 0) -> np.ndarray:
+    """Double-double permanent as ``[re_hi, re_lo, im_hi, im_lo]``."""
+    a, r, c = _prep(matrix, rows, cols)
+    out = np.zeros(4)
+    _check(
+        lib().pqo_perm_dd_c128(
+            _ptr(a.view(np.float64), ctypes.c_double), a.shape[0], a.shape[1],
+            _ptr(r, ctypes.c_int), _ptr(c, ctypes.c_int), n_jobs,
+            _ptr(out, ctypes.c_double),
+        )
+    )
+    return out
+
+
+def perm_dd(matrix, rows, cols, n_jobs: int = 0) -> complex:
+    """Extended-precision permanent rounded to complex128."""
+    p = perm_dd_parts(matrix, rows, cols, n_jobs)
+    return complex(p[0] + p[1], p[2] + p[3])
+
+
+def grad_dd(matrix, rows, cols, n_jobs: int = 0) -> np.ndarray:
+    """d perm / d A_ij = r_i c_j perm(A; r - e_i, c - e_j), double-double."""
+    a, r, c = _prep(matrix, rows, cols)
+    g = np.zeros(a.shape, dtype=np.complex128)
+    _check(
+        lib().pqo_grad_dd_c128(
+            _ptr(a.view(np.float64), ctypes.c_double), a.shape[0], a.shape[1],
+            _ptr(r, ctypes.c_int), _ptr(c, ctypes.c_int), n_jobs,
+            _ptr(g.view(np.float64), ctypes.c_double),
+        )
+    )
+    return g
+
+
+def gray_states(limits, offset: int = 0, count: Optional[int] = None) -> np.ndarray:
+    """Gray states ``offset .. offset+count-1`` of the reflected mixed-radix
+    counter with the given radices (digit 0 least significant)."""
+    lim = np.ascontiguousarray(np.asarray(limits, dtype=np.int32))
+    total = int(np.prod(lim, dtype=np.int64)) if lim.size else 1
+    if count is None:
+        count = total - offset
+    out = np.zeros((count, lim.size), dtype=np.int32)
+    st = lib().pqo_gray_states(
+        _ptr(lim, ctypes.c_int), lim.size, offset, count, _ptr(out, ctypes.c_int)
+    )
+    if st != 0:
+        raise ValueError("invalid Gray-state request")
+    return out
+
+
+def num_procs() -> int:
+    return int(lib().pqo_num_procs())
+
+
+# --------------------------------------------------------------------------
+# Independent formulas (pure Python / numpy)
+# --------------------------------------------------------------------------
+
+
+def assym_reduce(array, row_reduce_on, col_reduce_on):
+    """Repeat row i ``row_reduce_on[i]`` times and column j
+    ``col_reduce_on[j]`` times (piquasso/_math/linalg.py:98-117)."""
+    ri = np.repeat(np.arange(len(row_reduce_on)), np.asarray(row_reduce_on, int))
+    ci = np.repeat(np.arange(len(col_reduce_on)), np.asarray(col_reduce_on, int))
+    return np.asarray(array)[np.ix_(ri, ci)]
+
+
+def perm_bruteforce(matrix, rows=None, cols=None, exact: bool = False):
+    """Permanent by the definition, sum over all permutations of S_n.
+
+    With ``exact=True`` every entry is converted to a pair of Fractions (the
+    exact value of the double), so the result is the exact permanent of the
+    given floating-point matrix, returned as (Fraction re, Fraction im)."""
+    a = np.asarray(matrix)
+    if rows is not None:
+        a = assym_reduce(a, rows, cols)
+    n = a.shape[0]
+    if a.shape[1] != n:
+        raise ValueError("expanded matrix must be square")
+    if n == 0:
+        return (Fraction(1), Fraction(0)) if exact else 1.0 + 0j
+    if exact:
+        fr = [[(Fraction(float(np.real(a[i, j]))), Fraction(float(np.imag(a[i, j]))))
+               for j in range(n)] for i in range(n)]
+        tr, ti = Fraction(0), Fraction(0)
+        for p in itertools.permutations(range(n)):
+            pr, pi = Fraction(1), Fraction(0)
+            for i in range(n):
+                xr, xi = fr[i][p[i]]
+                pr, pi = pr * xr - pi * xi, pr * xi + pi * xr
+            tr += pr
+            ti += pi
+        return tr, ti
+    total = 0j
+    for p in itertools.permutations(range(n)):
+        prod = 1 + 0j
+        for i in range(n):
+            prod *= complex(a[i, p[i]])
+        total += prod
+    return total
+
+
+def perm_ryser(matrix, rows=None, cols=None) -> complex:
+    """Ryser's formula perm(A) = (-1)^n sum_S (-1)^|S| prod_i sum_{j in S} a_ij
+    over column subsets S (vectorised in numpy; n <= ~18)."""
+    a = np.asarray(matrix, dtype=np.complex128)
+    if rows is not None:
+        a = assym_reduce(a, rows, cols)
+    n = a.shape[0]
+    if n == 0:
+        return 1.0 + 0j
+    masks = np.arange(1, 1 << n, dtype=np.int64)
+    bits = ((masks[:, None] >> np.arange(n)[None, :]) & 1).astype(np.float64)
+    sizes = bits.sum(axis=1)
+    total = 0j
+    step = 1 << 14
+    for s in range(0, masks.size, step):
+        sub = bits[s : s + step]
+        rs = sub @ a.T  # (k, n): row sums over chosen columns, per row i
+        prod = np.prod(rs, axis=1)
+        sign = np.where((n - sizes[s : s + step]) % 2 == 0, 1.0, -1.0)
+        total += np.sum(sign * prod)
+    return complex(total)
+
+
+# --------------------------------------------------------------------------
+# The reference's own build (oracle/_ref, produced by `make -C oracle ref`)
+# --------------------------------------------------------------------------
+
+_ref_mod = None
+_ref_lib = None
+
+
+def ref_available() -> bool:
+    return os.path.exists(os.path.join(REF_DIR, "libpqref.so"))
+
+
+def ref_module():
+    """The reference pybind module ``permanent`` built from its sources."""
+    global _ref_mod
+    if _ref_mod is None:
+        path = os.path.join(REF_DIR, "permanent" + sysconfig.get_config_var("EXT_SUFFIX"))
+        if not os.path.exists(path):
+            raise OracleUnavailable("oracle/_ref has no reference build")
+        spec = importlib.util.spec_from_file_location("permanent", path)
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        _ref_mod = mod
+    return _ref_mod
+
+
+def ref_lib():
+    """ctypes handle of the shim around the reference C++ functions."""
+    global _ref_lib
+    if _ref_lib is None:
+        path = os.path.join(REF_DIR, "libpqref.so")
+        if not os.path.exists(path):
+            raise OracleUnavailable("oracle/_ref has no reference build")
+        L = ctypes.CDLL(path)
+        dp = ctypes.POINTER(ctypes.c_double)
+        fp = ctypes.POINTER(ctypes.c_float)
+        ip = ctypes.POINTER(ctypes.c_int)
+        ci = ctypes.c_int
+        L.pqref_perm_c128.argtypes = [dp, ci, ci, ip, ip, dp]
+        L.pqref_perm_c64.argtypes = [fp, ci, ci, ip, ip, fp]
+        L.pqref_grad_c128.argtypes = [dp, ci, ip, ip, dp]
+        L.pqref_laplace_c128.argtypes = [dp, ci, ci, ip, ip, dp]
+        for f in (L.pqref_perm_c128, L.pqref_perm_c64, L.pqref_grad_c128,
+                  L.pqref_laplace_c128):
+            f.restype = ci
+        _ref_lib = L
+    return _ref_lib
+
+
+def ref_perm(matrix, rows, cols) -> complex:
+    """``permanent_cpp<double>`` of the reference, called directly."""
+    a, r, c = _prep(matrix, rows, cols)
+    out = np.zeros(2)
+    _check(
+        ref_lib().pqref_perm_c128(
+            _ptr(a.view(np.float64), ctypes.c_double), a.shape[0], a.shape[1],
+            _ptr(r, ctypes.c_int), _ptr(c, ctypes.c_int), _ptr(out, ctypes.c_double),
+        )
+    )
+    return complex(out[0], out[1])
+
+
+def ref_grad(matrix, rows, cols) -> np.ndarray:
+    """``grad_perm`` of the reference (square matrices only)."""
+    a, r, c = _prep(matrix, rows, cols)
+    if a.shape[0] != a.shape[1]:
+        raise ValueError("grad_perm assumes a square matrix")
+    g = np.zeros(a.shape, dtype=np.complex128)
+    _check(
+        ref_lib().pqref_grad_c128(
+            _ptr(a.view(np.float64), ctypes.c_double), a.shape[0],
+            _ptr(r, ctypes.c_int), _ptr(c, ctypes.c_int),
+            _ptr(g.view(np.float64), ctypes.c_double),
+        )
+    )
+    return g

@@ -1,0 +1,16 @@
+"""MI355X-native permanent engine: a drop-in for piquasso's
+``permanent(matrix, rows, cols)`` (piquasso/_math/permanent.cpp:71-78) whose
+Glynn / Gray-code sweep runs in hand-written HIP kernels for gfx950.
+
+Public entry points
+-------------------
+permanent(matrix, rows, cols)        drop-in for piquasso._math.permanent.permanent
+permanent_ex(matrix, rows, cols, mode) one permanent with the double-double parts
+PermanentBatch / permanent_batch     many uneven permanents in one persistent launch
+"""
+
+from ._math.permanent import permanent
+from .engine import PermanentBatch, permanent_batch, permanent_ex
+
+__all__ = ["permanent", "permanent_ex", "PermanentBatch", "permanent_batch"]
+__version__ = "0.1.0"

@@ -1,0 +1,95 @@
+"""Drop-in replacement for piquasso's native module ``piquasso._math.permanent``.
+
+``permanent(matrix, rows, cols)`` mirrors the pybind entry of the reference
+(piquasso/_math/permanent.cpp:26-41,71-78): the same keyword names, the
+same dtype dispatch, a 0-d ``numpy.ndarray`` result (src/numpy_utils.hpp:36-49),
+1 for empty / all-zero multiplicities (src/permanent.cpp:106-108) and
+``RuntimeError`` when the row and column multiplicities do not sum to the same
+number (src/permanent.cpp:100-104).  The value is computed by the HIP kernels
+of ``libpqp_permanent.so``; there is no CPU path.
+
+Dtype dispatch, as the reference module built with pybind11 3.0 / numpy 2
+resolves its two overloads (``complex<float>`` registered before
+``complex<double>``, both ``c_style``, no forcecast):
+
+* a numpy array whose dtype casts *safely* to complex64 (complex64, float32,
+  float16, int8/16, uint8, bool) takes the complex64 path and returns
+  complex64;
+* otherwise a numpy array that casts safely to complex128 (complex128,
+  float64, int32/64, ...) takes the complex128 path, also when it is not
+  C-contiguous (it is copied);
+* any other array dtype is a ``TypeError``;
+* non-array inputs (lists, tuples) are converted to complex64.
+
+Multiplicities are cast to int32 like ``py::array_t<int, forcecast>``.
+
+Deliberate differences, where the reference has undefined behaviour: a
+``rows``/``cols`` length that does not match the matrix shape, a non-2-D
+matrix, or a negative multiplicity raise ``ValueError`` instead of reading
+out of bounds.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from .. import _capi
+
+__all__ = ["permanent", "resolve_dtype"]
+
+
+def resolve_dtype(matrix):
+    """The complex dtype the reference's pybind overload resolution picks."""
+    if isinstance(matrix, np.ndarray):
+        if np.can_cast(matrix.dtype, np.complex64, "safe"):
+            return np.dtype(np.complex64)
+        if np.can_cast(matrix.dtype, np.complex128, "safe"):
+            return np.dtype(np.complex128)
+        raise TypeError(
+            "permanent(): incompatible function arguments; matrix dtype "
+            f"{matrix.dtype} converts to neither complex64 nor complex128"
+        )
+    return np.dtype(np.complex64)
+
+
+def _multiplicities(values, name, length):
+    arr = np.ascontiguousarray(np.asarray(values).astype(np.int32, casting="unsafe").reshape(-1))
+    if arr.shape[0] != length:
+        raise ValueError(
+            f"permanent(): {name} has {arr.shape[0]} entries but the matrix has {length}"
+        )
+    return arr
+
+
+def permanent(matrix, rows, cols):
+    """Permanent of ``matrix`` with row multiplicities ``rows`` and column
+    multiplicities ``cols`` (Glynn's formula over the reflected n-ary Gray
+    code, src/permanent.cpp:49-264), evaluated on the GPU."""
+    dtype = resolve_dtype(matrix)
+    a = np.ascontiguousarray(np.asarray(matrix, dtype=dtype))
+    if a.ndim != 2:
+        raise ValueError(f"permanent(): matrix must be 2-D, got shape {a.shape}")
+    m_r, m_c = a.shape
+    r = _multiplicities(rows, "rows", m_r)
+    c = _multiplicities(cols, "cols", m_c)
+    L = _capi.lib()
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    if dtype == np.complex128:
+        out = np.zeros(2, dtype=np.float64)
+        st = L.pqp_permanent_c128(
+            a.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            m_r, m_c, m_c, r.ctypes.data_as(i32p), c.ctypes.data_as(i32p), 0,
+            out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), None,
+        )
+        _capi.check(st)
+        return np.array(complex(out[0], out[1]), dtype=np.complex128)
+    out = np.zeros(2, dtype=np.float32)
+    st = L.pqp_permanent_c64(
+        a.view(np.float32).ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+        m_r, m_c, m_c, r.ctypes.data_as(i32p), c.ctypes.data_as(i32p), 0,
+        out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), None,
+    )
+    _capi.check(st)
+    return np.array(complex(out[0], out[1]), dtype=np.complex64)

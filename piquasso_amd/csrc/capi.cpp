@@ -1,0 +1,487 @@
+// C ABI of the permanent engine (include/pqp_permanent.h).
+//
+// Host side of every call: plan (plan.cpp), copy the arena to the device,
+// launch the sweep kernels (perm_fwd.hip) and the finalize kernel on a HIP
+// stream, copy the results back.  Errors become status codes plus a
+// thread-local message; nothing throws across the ABI.
+#include "pqp_permanent.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "perm_kernels.hpp"
+#include "plan.hpp"
+#include "pqp_types.hpp"
+
+namespace {
+
+using namespace pqp;
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string &msg) {
+  g_error = msg;
+  return code;
+}
+
+#define PQP_HIP(expr)                                                               \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      return fail(PQP_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------------------- device --
+
+struct DeviceState {
+  bool ok = false;
+  int n_devices = 0;
+  int device = 0;
+  int cus = 0;
+  std::string name;
+  std::string error;
+  // occupancy cache: [bucket][fp32][comp][persistent], -1 = not queried
+  int occ[8][2][2][2];
+};
+
+DeviceState &device_state() {
+  static DeviceState st;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    std::memset(st.occ, 0xff, sizeof(st.occ));
+    hipError_t e = hipGetDeviceCount(&st.n_devices);
+    if (e != hipSuccess || st.n_devices < 1) {
+      st.error = std::string("no HIP device available (") +
+                 (e != hipSuccess ? hipGetErrorString(e) : "device count is 0") + ")";
+      st.n_devices = 0;
+      return;
+    }
+    if (hipGetDevice(&st.device) != hipSuccess) st.device = 0;
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, st.device);
+    if (e != hipSuccess) {
+      st.error = std::string("hipGetDeviceProperties: ") + hipGetErrorString(e);
+      return;
+    }
+    st.cus = prop.multiProcessorCount;
+    st.name = prop.gcnArchName;
+    st.ok = true;
+  });
+  return st;
+}
+
+// Work units per CU a launch is split into: about 3 resident waves per SIMD
+// (12 per CU) plus slack for load balance.
+constexpr int64_t kUnitsPerCu = 48;
+
+int require_device() {
+  DeviceState &d = device_state();
+  if (!d.ok) return fail(PQP_ERR_DEVICE, d.error);
+  return PQP_OK;
+}
+
+int blocks_per_cu(int bucket, bool fp32, bool comp, bool persistent) {
+  DeviceState &d = device_state();
+  int &slot = d.occ[bucket][fp32][comp][persistent];
+  if (slot < 0) {
+    int b = 0;
+    if (occupancy_fwd(bucket, persistent, fp32, comp, &b) != hipSuccess || b < 1) b = 1;
+    slot = b;
+  }
+  return slot;
+}
+
+// -------------------------------------------------------- device buffers --
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t n) {
+    if (n <= bytes) return PQP_OK;
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+      bytes = 0;
+    }
+    size_t alloc = std::max<size_t>(n, 4096);
+    hipError_t e = hipMalloc(&p, alloc);
+    if (e != hipSuccess) return fail(PQP_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    bytes = alloc;
+    return PQP_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+// Device image of one plan plus the stream it runs on.
+struct Exec {
+  Plan plan;
+  std::vector<int32_t> item_first_unit;  // first unit of each item in plan.units
+  DevBuf arena, partials, out, out_dd, heads;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  uint32_t flags = 0;
+
+  LaunchArgs args() const {
+    LaunchArgs a;
+    a.arena = static_cast<const char *>(arena.p);
+    a.items = reinterpret_cast<const ItemDesc *>(static_cast<const char *>(arena.p) + plan.items_off);
+    a.units = reinterpret_cast<const WorkUnit *>(static_cast<const char *>(arena.p) + plan.units_off);
+    a.partials = static_cast<double *>(partials.p);
+    a.out = static_cast<double *>(out.p);
+    a.out_dd = static_cast<double *>(out_dd.p);
+    a.queue_head = static_cast<int *>(heads.p);
+    a.n_units = static_cast<int32_t>(plan.units.size());
+    a.n_items = static_cast<int32_t>(plan.items.size());
+    return a;
+  }
+};
+
+void index_units(Exec &x) {
+  x.item_first_unit.assign(x.plan.items.size(), -1);
+  for (size_t u = 0; u < x.plan.units.size(); ++u) {
+    const WorkUnit &w = x.plan.units[u];
+    if (x.item_first_unit[w.item] < 0) x.item_first_unit[w.item] = static_cast<int32_t>(u);
+  }
+}
+
+// Upload the plan (allocating device buffers as needed).
+int upload(Exec &x) {
+  const Plan &p = x.plan;
+  const size_t n_items = std::max<size_t>(p.items.size(), 1);
+  int st;
+  if ((st = x.arena.ensure(p.arena.size())) != PQP_OK) return st;
+  if ((st = x.partials.ensure(std::max<int64_t>(p.n_partials, 1) * 4 * sizeof(double))) != PQP_OK) return st;
+  if ((st = x.out.ensure(n_items * 2 * sizeof(double))) != PQP_OK) return st;
+  if ((st = x.out_dd.ensure(n_items * 4 * sizeof(double))) != PQP_OK) return st;
+  if ((st = x.heads.ensure(64 * sizeof(int))) != PQP_OK) return st;
+  PQP_HIP(hipMemcpyAsync(x.arena.p, p.arena.data(), p.arena.size(), hipMemcpyHostToDevice, x.stream));
+  PQP_HIP(hipMemsetAsync(x.heads.p, 0, 64 * sizeof(int), x.stream));
+  return PQP_OK;
+}
+
+// Enqueue one evaluation.  ev_begin / ev_end (optional) bracket the sweep
+// kernels only when kernel_only is set, the whole evaluation otherwise.
+int enqueue(Exec &x, int mode, hipEvent_t ev_begin, hipEvent_t ev_end) {
+  const Plan &p = x.plan;
+  const LaunchArgs a = x.args();
+  const int n_items = static_cast<int>(p.items.size());
+  const bool kernel_only = (mode == PQP_RUN_KERNEL);
+  if (ev_begin) PQP_HIP(hipEventRecord(ev_begin, x.stream));
+  if (mode == PQP_RUN_PLAIN) {
+    for (int k = 0; k < n_items; ++k) {
+      const ItemDesc &d = p.items[k];
+      if (d.units > 0) {
+        FwdLaunch l;
+        l.args = a;
+        l.first_unit = x.item_first_unit[k];
+        l.n_units = d.units;
+        l.persistent = false;
+        l.fp32 = p.fp32;
+        l.comp = p.fp32;  // FP32 products always accumulate in dd
+        l.stream = x.stream;
+        int b = 0, mcp = 0;
+        b = bucket_for_width(d.mcp, p.fp32, &mcp);
+        PQP_HIP(launch_fwd(b, l));
+      }
+      PQP_HIP(launch_finalize(p.fp32, a, k, 1, 0, x.stream));
+    }
+  } else {
+    const bool fixed = (x.flags & PQP_FLAG_STATIC) != 0;
+    const DeviceState &dev = device_state();
+    int slot = 0;
+    for (const Plan::Range &r : p.ranges) {
+      FwdLaunch l;
+      l.args = a;
+      l.first_unit = r.first;
+      l.n_units = r.count;
+      l.head_slot = slot++;
+      l.persistent = !fixed;
+      l.fp32 = p.fp32;
+      l.comp = !(x.flags & PQP_FLAG_PLAIN) || p.fp32;
+      l.stream = x.stream;
+      if (l.persistent) {
+        const int per_cu = blocks_per_cu(r.bucket, l.fp32, l.comp, true);
+        l.grid = std::max(1, std::min(r.count, dev.cus * per_cu));
+      }
+      PQP_HIP(launch_fwd(r.bucket, l));
+    }
+    if (kernel_only && ev_end) PQP_HIP(hipEventRecord(ev_end, x.stream));
+    const bool comp = !(x.flags & PQP_FLAG_PLAIN) || p.fp32;
+    PQP_HIP(launch_finalize(comp, a, 0, n_items, static_cast<int>(p.ranges.size()), x.stream));
+  }
+  if (ev_end && !kernel_only) PQP_HIP(hipEventRecord(ev_end, x.stream));
+  return PQP_OK;
+}
+
+}  // namespace
+
+struct pqp_batch {
+  Exec x;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  bool ran = false;
+};
+
+namespace {
+
+// Shared state of the single-permanent entry points: buffers grow and are
+// reused across calls; calls serialise on a mutex.
+struct SingleState {
+  std::mutex mu;
+  Exec x;
+  bool init = false;
+};
+
+SingleState &single_state() {
+  static SingleState *s = new SingleState();  // never destroyed: outlives atexit HIP teardown
+  return *s;
+}
+
+int run_single(const PermInput &in, uint32_t flags, double *out2, double *out_dd4) {
+  // Validate and plan first: argument errors are reported the same way with
+  // or without a device.
+  PlanOptions opt;
+  opt.fp32_products = (flags & PQP_FLAG_FP32) != 0;
+  const DeviceState &dev = device_state();
+  // enough waves to fill every CU several times over for large permanents
+  opt.min_units = static_cast<int64_t>(std::max(dev.cus, 1)) * kUnitsPerCu;
+  Plan plan;
+  std::string err;
+  int st = plan_build(std::vector<PermInput>{in}, opt, plan, err);
+  if (st != PQP_OK) return fail(st, err);
+  if ((st = require_device()) != PQP_OK) return st;
+  SingleState &S = single_state();
+  std::lock_guard<std::mutex> lock(S.mu);
+  Exec &x = S.x;
+  if (!S.init) {
+    PQP_HIP(hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking));
+    x.own_stream = true;
+    S.init = true;
+  }
+  x.plan = std::move(plan);
+  index_units(x);
+  x.flags = (flags & PQP_FLAG_PLAIN) | PQP_FLAG_STATIC;
+  if ((st = upload(x)) != PQP_OK) return st;
+  if ((st = enqueue(x, (flags & PQP_FLAG_PLAIN) ? PQP_RUN_PLAIN : PQP_RUN_EAGER, nullptr,
+                    nullptr)) != PQP_OK)
+    return st;
+  double res[6];
+  PQP_HIP(hipMemcpyAsync(res, x.out.p, 2 * sizeof(double), hipMemcpyDeviceToHost, x.stream));
+  PQP_HIP(hipMemcpyAsync(res + 2, x.out_dd.p, 4 * sizeof(double), hipMemcpyDeviceToHost, x.stream));
+  PQP_HIP(hipStreamSynchronize(x.stream));
+  out2[0] = res[0];
+  out2[1] = res[1];
+  if (out_dd4) std::memcpy(out_dd4, res + 2, 4 * sizeof(double));
+  return PQP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *pqp_version(void) { return "pqp-permanent 0.1.0 (gfx950)"; }
+
+const char *pqp_last_error(void) { return g_error.c_str(); }
+
+int pqp_device_info(int *n_devices, int *cu_count, char *name, int name_len) {
+  DeviceState &d = device_state();
+  if (n_devices) *n_devices = d.n_devices;
+  if (!d.ok) return fail(PQP_ERR_DEVICE, d.error);
+  if (cu_count) *cu_count = d.cus;
+  if (name && name_len > 0) {
+    std::strncpy(name, d.name.c_str(), static_cast<size_t>(name_len) - 1);
+    name[name_len - 1] = '\0';
+  }
+  return PQP_OK;
+}
+
+int pqp_permanent_c128(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
+                       const int32_t *rows, const int32_t *cols, uint32_t flags, double *out,
+                       double *out_dd) {
+  if (!out) return fail(PQP_ERR_INVALID, "out is null");
+  PermInput in{matrix, false, m_r, m_c, ld, rows, cols};
+  return run_single(in, flags & (PQP_FLAG_PLAIN | PQP_FLAG_FP32), out, out_dd);
+}
+
+int pqp_permanent_c64(const float *matrix, int64_t m_r, int64_t m_c, int64_t ld,
+                      const int32_t *rows, const int32_t *cols, uint32_t flags, float *out,
+                      double *out_dd) {
+  if (!out) return fail(PQP_ERR_INVALID, "out is null");
+  (void)flags;
+  PermInput in{matrix, true, m_r, m_c, ld, rows, cols};
+  double o[2];
+  const int st = run_single(in, PQP_FLAG_FP32, o, out_dd);
+  if (st != PQP_OK) return st;
+  out[0] = static_cast<float>(o[0]);
+  out[1] = static_cast<float>(o[1]);
+  return PQP_OK;
+}
+
+int pqp_batch_create(int64_t count, const void *const *matrices, int is_c64, const int64_t *m_r,
+                     const int64_t *m_c, const int64_t *ld, const int32_t *const *rows,
+                     const int32_t *const *cols, uint32_t flags, int64_t target_inner,
+                     pqp_batch **out) {
+  if (!out) return fail(PQP_ERR_INVALID, "out is null");
+  *out = nullptr;
+  if (count < 0 || (count > 0 && (!matrices || !m_r || !m_c || !ld || !rows || !cols)))
+    return fail(PQP_ERR_INVALID, "invalid batch arguments");
+  if (count > INT32_MAX) return fail(PQP_ERR_RANGE, "too many items");
+  std::vector<PermInput> inputs(static_cast<size_t>(count));
+  for (int64_t k = 0; k < count; ++k)
+    inputs[k] = PermInput{matrices[k], is_c64 != 0, m_r[k], m_c[k], ld[k], rows[k], cols[k]};
+  std::unique_ptr<pqp_batch> b(new (std::nothrow) pqp_batch());
+  if (!b) return fail(PQP_ERR_NOMEM, "out of host memory");
+  PlanOptions opt;
+  opt.fp32_products = (flags & PQP_FLAG_FP32) != 0 || is_c64 != 0;
+  opt.target_inner = target_inner;
+  // about kUnitsPerCu waves per CU over the whole batch, split by cost
+  opt.total_units = static_cast<int64_t>(std::max(device_state().cus, 1)) * kUnitsPerCu;
+  std::string err;
+  int st = plan_build(inputs, opt, b->x.plan, err);
+  if (st != PQP_OK) return fail(st, err);
+  if ((st = require_device()) != PQP_OK) return st;
+  index_units(b->x);
+  b->x.flags = flags;
+  PQP_HIP(hipStreamCreateWithFlags(&b->x.stream, hipStreamNonBlocking));
+  b->x.own_stream = true;
+  if ((st = upload(b->x)) != PQP_OK) {
+    pqp_batch_destroy(b.release());
+    return st;
+  }
+  PQP_HIP(hipStreamSynchronize(b->x.stream));
+  *out = b.release();
+  return PQP_OK;
+}
+
+int pqp_batch_run(pqp_batch *b, int mode) {
+  if (!b) return fail(PQP_ERR_INVALID, "batch is null");
+  int st;
+  if (mode == PQP_RUN_GRAPH) {
+    if (!b->graph_exec) {
+      PQP_HIP(hipStreamBeginCapture(b->x.stream, hipStreamCaptureModeThreadLocal));
+      st = enqueue(b->x, PQP_RUN_EAGER, nullptr, nullptr);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(b->x.stream, &g);
+      if (st != PQP_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+      }
+      if (e != hipSuccess) return fail(PQP_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      b->graph = g;
+      PQP_HIP(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+    }
+    PQP_HIP(hipGraphLaunch(b->graph_exec, b->x.stream));
+  } else if (mode == PQP_RUN_EAGER || mode == PQP_RUN_PLAIN || mode == PQP_RUN_KERNEL) {
+    if ((st = enqueue(b->x, mode, nullptr, nullptr)) != PQP_OK) return st;
+  } else {
+    return fail(PQP_ERR_INVALID, "unknown run mode");
+  }
+  b->ran = true;
+  return PQP_OK;
+}
+
+int pqp_batch_sync(pqp_batch *b) {
+  if (!b) return fail(PQP_ERR_INVALID, "batch is null");
+  PQP_HIP(hipStreamSynchronize(b->x.stream));
+  return PQP_OK;
+}
+
+int pqp_batch_results(pqp_batch *b, double *out, double *out_dd) {
+  if (!b) return fail(PQP_ERR_INVALID, "batch is null");
+  if (!b->ran) return fail(PQP_ERR_INVALID, "batch has not been run");
+  const size_t n = b->x.plan.items.size();
+  PQP_HIP(hipStreamSynchronize(b->x.stream));
+  if (out && n) PQP_HIP(hipMemcpy(out, b->x.out.p, n * 2 * sizeof(double), hipMemcpyDeviceToHost));
+  if (out_dd && n)
+    PQP_HIP(hipMemcpy(out_dd, b->x.out_dd.p, n * 4 * sizeof(double), hipMemcpyDeviceToHost));
+  return PQP_OK;
+}
+
+int pqp_batch_time(pqp_batch *b, int mode, int reps, float *ms) {
+  if (!b || reps < 0 || (reps > 0 && !ms)) return fail(PQP_ERR_INVALID, "invalid timing arguments");
+  hipEvent_t e0, e1;
+  PQP_HIP(hipEventCreate(&e0));
+  PQP_HIP(hipEventCreate(&e1));
+  int st = PQP_OK;
+  for (int r = 0; r < reps && st == PQP_OK; ++r) {
+    if (mode == PQP_RUN_GRAPH) {
+      if (!b->graph_exec) st = pqp_batch_run(b, PQP_RUN_GRAPH);  // builds the graph
+      if (st != PQP_OK) break;
+      (void)hipStreamSynchronize(b->x.stream);
+      (void)hipEventRecord(e0, b->x.stream);
+      (void)hipGraphLaunch(b->graph_exec, b->x.stream);
+      (void)hipEventRecord(e1, b->x.stream);
+    } else {
+      st = enqueue(b->x, mode, e0, e1);
+      if (st != PQP_OK) break;
+    }
+    hipError_t e = hipEventSynchronize(e1);
+    if (e != hipSuccess) {
+      st = fail(PQP_ERR_DEVICE, std::string("hipEventSynchronize: ") + hipGetErrorString(e));
+      break;
+    }
+    (void)hipEventElapsedTime(&ms[r], e0, e1);
+    b->ran = true;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return st;
+}
+
+int pqp_batch_stats(const pqp_batch *b, double *flops, int64_t *terms, int64_t *units,
+                    int64_t *arena_bytes) {
+  if (!b) return fail(PQP_ERR_INVALID, "batch is null");
+  if (flops) *flops = b->x.plan.total_flops;
+  if (terms) *terms = b->x.plan.total_terms;
+  if (units) *units = static_cast<int64_t>(b->x.plan.units.size());
+  if (arena_bytes) *arena_bytes = static_cast<int64_t>(b->x.plan.arena.size());
+  return PQP_OK;
+}
+
+int pqp_plan_image(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
+                   const int32_t *rows, const int32_t *cols, uint32_t flags,
+                   int64_t target_inner, int64_t min_units, void *buf, int64_t buf_bytes,
+                   int64_t *needed_bytes) {
+  PermInput in{matrix, false, m_r, m_c, ld, rows, cols};
+  PlanOptions opt;
+  opt.fp32_products = (flags & PQP_FLAG_FP32) != 0;
+  opt.target_inner = target_inner;
+  opt.min_units = min_units;
+  Plan p;
+  std::string err;
+  const int st = plan_build(std::vector<PermInput>{in}, opt, p, err);
+  if (st != PQP_OK) return fail(st, err);
+  if (needed_bytes) *needed_bytes = static_cast<int64_t>(p.arena.size());
+  if (buf && buf_bytes >= static_cast<int64_t>(p.arena.size()))
+    std::memcpy(buf, p.arena.data(), p.arena.size());
+  else if (buf)
+    return fail(PQP_ERR_INVALID, "buffer too small for the plan image");
+  return PQP_OK;
+}
+
+void pqp_batch_destroy(pqp_batch *b) {
+  if (!b) return;
+  if (b->x.stream) (void)hipStreamSynchronize(b->x.stream);
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->x.arena.release();
+  b->x.partials.release();
+  b->x.out.release();
+  b->x.out_dd.release();
+  b->x.heads.release();
+  if (b->x.own_stream && b->x.stream) (void)hipStreamDestroy(b->x.stream);
+  delete b;
+}
+
+}  // extern "C"

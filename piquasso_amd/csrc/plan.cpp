@@ -1,0 +1,476 @@
+// Host-side planning for the permanent kernels (see plan.hpp).
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace pqp {
+
+namespace {
+
+constexpr int64_t kMaxTerms = int64_t(1) << 62;
+constexpr int64_t kMaxInner = int64_t(1) << 16;   // step/coef table length cap
+
+int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+int64_t binom64(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  if (k > n - k) k = n - k;
+  int64_t r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+  return r;
+}
+
+// Append `bytes` of zeroed space to the arena at 256-byte alignment.
+int64_t arena_alloc(std::vector<char> &arena, int64_t bytes) {
+  int64_t off = align_up(static_cast<int64_t>(arena.size()), 256);
+  arena.resize(static_cast<size_t>(off + std::max<int64_t>(bytes, 1)), 0);
+  return off;
+}
+
+template <typename T>
+T *at(std::vector<char> &arena, int64_t off) {
+  return reinterpret_cast<T *>(arena.data() + off);
+}
+
+// One digit of the Gray code: a matrix row (compacted columns) and its
+// multiplicity rho >= 1 (radix rho + 1).
+struct Digit {
+  int64_t src_row;
+  int rho;
+};
+
+// Shared inner tables, keyed by the inner multiplicity profile.
+struct InnerTables {
+  int64_t step_off = 0;
+  int64_t coef_off = 0;
+};
+
+}  // namespace
+
+const std::vector<Bucket> &buckets() {
+  static const std::vector<Bucket> b = {
+      {0, 1, 8},   {1, 9, 16},  {2, 17, 24}, {3, 25, 32},
+      {4, 40, 40}, {5, 48, 48}, {6, 56, 56}, {7, 64, 64},
+  };
+  return b;
+}
+
+int bucket_for_width(int mc, bool /*fp32*/, int *mcp_out) {
+  if (mc < 1 || mc > kMaxCols) return -1;
+  int mcp = mc <= 32 ? mc : static_cast<int>(align_up(mc, 8));
+  for (const Bucket &b : buckets())
+    if (mcp >= b.lo && mcp <= b.hi) {
+      *mcp_out = mcp;
+      return b.id;
+    }
+  return -1;
+}
+
+void gray_steps(const std::vector<int> &radix, std::vector<int> &digit,
+                std::vector<int> &value) {
+  const int nd = static_cast<int>(radix.size());
+  int64_t total = 1;
+  for (int r : radix) total *= r;
+  digit.assign(static_cast<size_t>(std::max<int64_t>(total - 1, 0)), 0);
+  value.assign(digit.size(), 0);
+  std::vector<int> ctr(nd, 0), gray(nd, 0);
+  for (int64_t s = 0; s + 1 < total; ++s) {
+    // increment the mixed-radix counter with carry
+    int idx = 0;
+    while (ctr[idx] == radix[idx] - 1) ctr[idx++] = 0;
+    ctr[idx]++;
+    // top-down: Gray digit = counter digit, reflected when the parity of the
+    // higher Gray digits is odd; the first mismatch from the top is the move
+    int parity = 0;
+    for (int i = nd - 1; i >= 0; --i) {
+      int v = parity ? radix[i] - 1 - ctr[i] : ctr[i];
+      parity ^= (v & 1);
+      if (v != gray[i]) {
+        digit[s] = i;
+        value[s] = v;
+        gray[i] = v;
+        break;
+      }
+    }
+  }
+}
+
+int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
+               Plan &plan, std::string &err) {
+  plan = Plan();
+  plan.fp32 = opt.fp32_products;
+  const int64_t n_items = static_cast<int64_t>(inputs.size());
+  const size_t rsz = opt.fp32_products ? sizeof(float) : sizeof(double);
+
+  plan.items.resize(n_items);
+  plan.info.resize(n_items);
+  plan.items_off = arena_alloc(plan.arena, n_items * int64_t(sizeof(ItemDesc)));
+
+  std::map<std::vector<int>, InnerTables> inner_cache;
+  std::vector<int64_t> item_units(n_items, 0);
+  std::vector<int> item_bucket(n_items, -1);
+  int64_t partial_cursor = 0;
+
+  // Pass 1: validate every item and estimate its cost, so that large items
+  // can be split into a share of the requested total number of work units
+  // proportional to their cost (load balance across uneven items).
+  std::vector<double> item_cost(n_items, 0.0);
+  double total_cost = 0.0;
+  for (int64_t k = 0; k < n_items; ++k) {
+    const PermInput &in = inputs[k];
+    if (in.m_r < 0 || in.m_c < 0 || (in.m_r * in.m_c > 0 && !in.a) ||
+        (in.m_r > 0 && !in.rows) || (in.m_c > 0 && !in.cols) ||
+        (in.m_r > 0 && in.ld < in.m_c)) {
+      err = "item " + std::to_string(k) + ": invalid shape or null pointer";
+      return kErrInvalid;
+    }
+    int64_t sr = 0, sc = 0, mc = 0;
+    for (int64_t i = 0; i < in.m_r; ++i) sr += std::max(in.rows[i], 0);
+    for (int64_t j = 0; j < in.m_c; ++j) {
+      sc += std::max(in.cols[j], 0);
+      mc += in.cols[j] > 0;
+    }
+    if (sr == 0 || sr != sc || sr > 64) continue;  // reported in pass 2
+    // T = prod (rho + 1) with the prepend applied to the first minimal row
+    int minelem = 0;
+    int64_t i0 = 0;
+    for (int64_t i = 0; i < in.m_r; ++i)
+      if (minelem == 0 || (in.rows[i] < minelem && in.rows[i] != 0)) {
+        minelem = in.rows[i];
+        i0 = i;
+      }
+    double T = 1.0;
+    for (int64_t i = 0; i < in.m_r; ++i) {
+      const int rho = in.rows[i] - (i == i0 ? 1 : 0);
+      if (rho > 0) T *= rho + 1;
+    }
+    item_cost[k] = T * (2.0 * mc + 4.0 * (sr - 1) + 8.0);
+    total_cost += item_cost[k];
+  }
+
+  // Pass 2: tables.
+  for (int64_t k = 0; k < n_items; ++k) {
+    const PermInput &in = inputs[k];
+    ItemDesc d;
+    std::memset(&d, 0, sizeof(d));
+    d.out_index = static_cast<int32_t>(k);
+    int64_t item_min_units = opt.min_units;
+    if (opt.total_units > 0 && total_cost > 0.0) {
+      const double share = item_cost[k] / total_cost * static_cast<double>(opt.total_units);
+      item_min_units = std::max<int64_t>(item_min_units, static_cast<int64_t>(std::ceil(share)));
+    }
+    if (in.m_r < 0 || in.m_c < 0 || (in.m_r * in.m_c > 0 && !in.a) ||
+        (in.m_r > 0 && !in.rows) || (in.m_c > 0 && !in.cols) ||
+        (in.m_r > 0 && in.ld < in.m_c)) {
+      err = "item " + std::to_string(k) + ": invalid shape or null pointer";
+      return kErrInvalid;
+    }
+    int64_t sr = 0, sc = 0;
+    for (int64_t i = 0; i < in.m_r; ++i) {
+      if (in.rows[i] < 0) {
+        err = "item " + std::to_string(k) + ": negative row multiplicity";
+        return kErrInvalid;
+      }
+      sr += in.rows[i];
+    }
+    for (int64_t j = 0; j < in.m_c; ++j) {
+      if (in.cols[j] < 0) {
+        err = "item " + std::to_string(k) + ": negative column multiplicity";
+        return kErrInvalid;
+      }
+      sc += in.cols[j];
+    }
+    if (sr != sc) {
+      err = "Number of input and output states should be equal";
+      return kErrSumMismatch;
+    }
+    if (sr == 0) {
+      // perm of an empty selection is 1 (src/permanent.cpp:106-108)
+      plan.info[k].trivial_one = true;
+      d.units = 0;
+      d.n = 0;
+      d.partial_base = partial_cursor;
+      plan.items[k] = d;
+      continue;
+    }
+    if (sr > 64) {
+      err = "item " + std::to_string(k) + ": more than 64 particles";
+      return kErrRange;
+    }
+    const int n = static_cast<int>(sr);
+
+    // --- compaction: columns with c_j > 0, sorted by multiplicity (desc) ---
+    std::vector<int64_t> colsel;
+    for (int64_t j = 0; j < in.m_c; ++j)
+      if (in.cols[j] > 0) colsel.push_back(j);
+    std::stable_sort(colsel.begin(), colsel.end(), [&](int64_t x, int64_t y) {
+      return in.cols[x] > in.cols[y];
+    });
+    const int mc = static_cast<int>(colsel.size());
+    int mcp = 0;
+    const int bucket = bucket_for_width(mc, opt.fp32_products, &mcp);
+    if (bucket < 0) {
+      err = "item " + std::to_string(k) + ": " + std::to_string(mc) +
+            " columns exceed the widest kernel (64)";
+      return kErrRange;
+    }
+
+    // --- prepend rule of src/permanent.cpp:54-95: the first row with the
+    //     minimal non-zero multiplicity becomes row 0 with multiplicity 1 ---
+    int64_t i0 = 0;
+    int minelem = 0;
+    for (int64_t i = 0; i < in.m_r; ++i)
+      if (minelem == 0 || (in.rows[i] < minelem && in.rows[i] != 0)) {
+        minelem = in.rows[i];
+        i0 = i;
+      }
+    std::vector<Digit> digits;
+    for (int64_t i = 0; i < in.m_r; ++i) {
+      int rho = in.rows[i] - (i == i0 ? 1 : 0);
+      if (rho > 0) digits.push_back(Digit{i, rho});
+    }
+    const int D = static_cast<int>(digits.size());
+    if (D > kMaxDigits) {
+      err = "item " + std::to_string(k) + ": too many Gray digits";
+      return kErrRange;
+    }
+    int64_t T = 1;
+    for (const Digit &g : digits) {
+      if (T > kMaxTerms / (g.rho + 1)) {
+        err = "item " + std::to_string(k) + ": Gray term count exceeds 2^62";
+        return kErrRange;
+      }
+      T *= (g.rho + 1);
+    }
+
+    // --- inner/top split: inner digits = smallest radices first ---
+    std::vector<int> order(D);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+      return digits[x].rho < digits[y].rho;
+    });
+    int cmax = 0;
+    for (int64_t j : colsel) cmax = std::max(cmax, static_cast<int>(in.cols[j]));
+    const double step_cost = 2.0 * mc + 4.0 * (n - 1) + 2.0 + 6.0 +
+                             (cmax > 1 ? 2.0 * (n - mc) : 0.0);
+    int best_L = 0;
+    double best_total = -1.0;
+    {
+      int64_t S = 1;
+      for (int L = 0; L <= D; ++L) {
+        if (L > 0) S *= (digits[order[L - 1]].rho + 1);
+        if (S > kMaxInner) break;
+        const int64_t U = T / S;
+        const int64_t units = (U + kWave - 1) / kWave;
+        const int Dt = D - L;
+        const double init_cost = (1.0 + D) * 2.0 * mcp + Dt * 40.0 + 400.0;
+        const double unit_cost = init_cost + static_cast<double>(S) * step_cost;
+        const double total = static_cast<double>(units) * unit_cost;
+        bool ok = true;
+        if (opt.target_inner > 0) {
+          // largest inner size not above the request
+          ok = (S <= opt.target_inner) || L == 0;
+          if (ok) {
+            best_L = L;
+            best_total = total;
+          }
+          continue;
+        }
+        if (item_min_units > 0 && units < item_min_units && L > 0) ok = false;
+        if (ok && (best_total < 0 || total < best_total)) {
+          best_total = total;
+          best_L = L;
+        }
+      }
+    }
+    const int L = best_L;
+    const int Dt = D - L;
+    std::vector<int> inner(order.begin(), order.begin() + L);
+    std::vector<int> top;
+    {
+      std::vector<char> is_inner(D, 0);
+      for (int x : inner) is_inner[x] = 1;
+      for (int x = 0; x < D; ++x)
+        if (!is_inner[x]) top.push_back(x);
+    }
+    int64_t S_in = 1;
+    for (int x : inner) S_in *= digits[x].rho + 1;
+    const int64_t U = T / S_in;
+    if (U >= (int64_t(1) << 50)) {
+      err = "item " + std::to_string(k) + ": too many top states";
+      return kErrRange;
+    }
+
+    // --- tables ---
+    const int rows_B = 1 + L + Dt;
+    d.b_off = arena_alloc(plan.arena, int64_t(rows_B) * mcp * 2 * rsz);
+    d.b2_off = arena_alloc(plan.arena, int64_t(2 * std::max(L, 1)) * mcp * 2 * rsz);
+    auto put = [&](int64_t off, int64_t row, int col, double re, double im) {
+      int64_t e = (row * mcp + col) * 2;
+      if (opt.fp32_products) {
+        at<float>(plan.arena, off)[e] = static_cast<float>(re);
+        at<float>(plan.arena, off)[e + 1] = static_cast<float>(im);
+      } else {
+        at<double>(plan.arena, off)[e] = re;
+        at<double>(plan.arena, off)[e + 1] = im;
+      }
+    };
+    auto entry = [&](int64_t row, int64_t col, double &re, double &im) {
+      const int64_t e = (row * in.ld + col) * 2;
+      if (in.a_is_float) {
+        const float *p = static_cast<const float *>(in.a);
+        re = p[e];
+        im = p[e + 1];
+      } else {
+        const double *p = static_cast<const double *>(in.a);
+        re = p[e];
+        im = p[e + 1];
+      }
+    };
+    // B rows: prepended row, inner digit rows, top digit rows
+    std::vector<int64_t> src_rows;
+    src_rows.push_back(i0);
+    for (int x : inner) src_rows.push_back(digits[x].src_row);
+    for (int x : top) src_rows.push_back(digits[x].src_row);
+    for (int r = 0; r < rows_B; ++r) {
+      for (int j = 0; j < mcp; ++j) {
+        double re = 0.0, im = 0.0;
+        if (j < mc)
+          entry(src_rows[r], colsel[j], re, im);
+        else if (r == 0)
+          re = 1.0;  // padding column: row sum stays exactly 1 + 0i
+        put(d.b_off, r, j, re, im);
+      }
+    }
+    // +-2B rows of the inner digits (doubling is exact)
+    for (int t = 0; t < L; ++t)
+      for (int j = 0; j < mc; ++j) {
+        double re, im;
+        entry(digits[inner[t]].src_row, colsel[j], re, im);
+        put(d.b2_off, 2 * t, j, 2.0 * re, 2.0 * im);
+        put(d.b2_off, 2 * t + 1, j, -2.0 * re, -2.0 * im);
+      }
+    // inner start coefficients alpha_k = rho_k (all inner g = 0)
+    d.alpha_off = arena_alloc(plan.arena, int64_t(std::max(L, 1)) * 8);
+    for (int t = 0; t < L; ++t)
+      at<double>(plan.arena, d.alpha_off)[t] = static_cast<double>(digits[inner[t]].rho);
+    // shared inner step / coefficient tables
+    std::vector<int> profile;
+    for (int x : inner) profile.push_back(digits[x].rho);
+    auto it = inner_cache.find(profile);
+    if (it == inner_cache.end()) {
+      InnerTables tb;
+      std::vector<int> radix;
+      for (int r : profile) radix.push_back(r + 1);
+      std::vector<int> sd, sv;
+      gray_steps(radix, sd, sv);
+      tb.step_off = arena_alloc(plan.arena, int64_t(std::max<size_t>(sd.size(), 1)) * 4);
+      tb.coef_off = arena_alloc(plan.arena, S_in * 8);
+      std::vector<int> g(L, 0);
+      int64_t w = 1;
+      at<double>(plan.arena, tb.coef_off)[0] = 1.0;
+      for (size_t s = 0; s < sd.size(); ++s) {
+        const int t = sd[s];
+        const int prev = g[t];
+        g[t] = sv[s];
+        // moving g up subtracts 2 b_t from the row sums (row 2t+1 = -2B),
+        // moving it down adds 2 b_t (row 2t)
+        at<uint32_t>(plan.arena, tb.step_off)[s] =
+            static_cast<uint32_t>(2 * t + (sv[s] > prev ? 1 : 0));
+        w = w / binom64(profile[t], prev) * binom64(profile[t], g[t]);
+        double sgn = ((s + 1) & 1) ? -1.0 : 1.0;
+        at<double>(plan.arena, tb.coef_off)[s + 1] = sgn * static_cast<double>(w);
+      }
+      it = inner_cache.emplace(profile, tb).first;
+    }
+    d.step_off = it->second.step_off;
+    d.coef_off = it->second.coef_off;
+    // top digits: {1 / place value, radix, rho, 0}
+    d.top_off = arena_alloc(plan.arena, int64_t(std::max(Dt, 1)) * 32);
+    bool top_binary = true;
+    {
+      int64_t place = 1;
+      for (int t = 0; t < Dt; ++t) {
+        const int rho = digits[top[t]].rho;
+        double *td = at<double>(plan.arena, d.top_off) + 4 * t;
+        td[0] = 1.0 / static_cast<double>(place);
+        td[1] = static_cast<double>(rho + 1);
+        td[2] = static_cast<double>(rho);
+        td[3] = 0.0;
+        place *= rho + 1;
+        if (rho != 1) top_binary = false;
+      }
+    }
+    // column multiplicity counts: cpow[r-2] = #{j : c_j >= r}, r = 2..cmax
+    d.cpow_off = arena_alloc(plan.arena, int64_t(std::max(cmax - 1, 1)) * 4);
+    for (int r = 2; r <= cmax; ++r) {
+      int cnt = 0;
+      for (int64_t j : colsel) cnt += in.cols[j] >= r;
+      at<int32_t>(plan.arena, d.cpow_off)[r - 2] = cnt;
+    }
+
+    d.S_in = S_in;
+    d.U = U;
+    d.units = static_cast<int32_t>((U + kWave - 1) / kWave);
+    d.mcp = mcp;
+    d.mc = mc;
+    d.n = n;
+    d.L = L;
+    d.Dt = Dt;
+    d.cmax = cmax;
+    d.flags = (cmax == 1 ? kColsAllOnes : 0) | (top_binary ? kTopBinary : 0);
+    d.partial_base = partial_cursor;
+    partial_cursor += d.units;
+    plan.items[k] = d;
+
+    ItemInfo &inf = plan.info[k];
+    inf.terms = T;
+    inf.flops = static_cast<double>(T) * (2.0 * mc + 6.0 * (n - 1) + 4.0);
+    inf.unit_cost = ((1.0 + D) * 2.0 * mcp + Dt * 40.0 + 400.0) +
+                    static_cast<double>(S_in) * step_cost;
+    plan.total_flops += inf.flops;
+    plan.total_terms += T;
+    item_units[k] = d.units;
+    item_bucket[k] = bucket;
+  }
+  plan.n_partials = partial_cursor;
+
+  // --- work units: grouped by kernel bucket, LPT (costliest units first) ---
+  std::vector<int64_t> idx(n_items);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) {
+    if (item_bucket[x] != item_bucket[y]) return item_bucket[x] < item_bucket[y];
+    return plan.info[x].unit_cost > plan.info[y].unit_cost;
+  });
+  int64_t total_units = 0;
+  for (int64_t k = 0; k < n_items; ++k) total_units += item_units[k];
+  if (total_units > INT32_MAX) {
+    err = "too many work units";
+    return kErrRange;
+  }
+  plan.units.reserve(static_cast<size_t>(total_units));
+  for (int64_t k : idx) {
+    if (item_units[k] == 0) continue;
+    const int b = item_bucket[k];
+    if (plan.ranges.empty() || plan.ranges.back().bucket != b)
+      plan.ranges.push_back(Plan::Range{b, static_cast<int32_t>(plan.units.size()), 0});
+    for (int64_t l = 0; l < item_units[k]; ++l) {
+      plan.units.push_back(WorkUnit{static_cast<int32_t>(k), static_cast<int32_t>(l)});
+      plan.bucket_of_unit.push_back(b);
+    }
+    plan.ranges.back().count += static_cast<int32_t>(item_units[k]);
+  }
+  plan.units_off = arena_alloc(plan.arena,
+                               int64_t(std::max<size_t>(plan.units.size(), 1)) * sizeof(WorkUnit));
+  if (!plan.units.empty())
+    std::memcpy(plan.arena.data() + plan.units_off, plan.units.data(),
+                plan.units.size() * sizeof(WorkUnit));
+  std::memcpy(plan.arena.data() + plan.items_off, plan.items.data(),
+              plan.items.size() * sizeof(ItemDesc));
+  return kOk;
+}
+
+}  // namespace pqp

@@ -1,0 +1,83 @@
+// Data layout shared by the host planner (plan.cpp) and the HIP kernels.
+//
+// A launch evaluates a batch of permanents.  The planner turns every
+// permanent into a Glynn sweep over a reflected mixed-radix Gray code
+// (the method of src/permanent.cpp:49-264, src/n_aryGrayCodeCounter.hpp):
+//
+//   perm = 2^-(n-1) * sum_g (-1)^{|g|} prod_i C(rho_i, g_i)
+//                    * prod_j ( b_0j + sum_i (rho_i - 2 g_i) b_ij )^{c_j}
+//
+// and splits the digit set into "inner" digits, swept in the same order by
+// all 64 lanes of a wavefront, and "top" digits, one top state per lane.
+// All lanes of a wave therefore change the same row with the same sign at
+// the same step, so row data, step descriptors and weights are wave-uniform
+// (scalar loads), and only the m_c complex row sums live in per-lane VGPRs.
+//
+// Everything the kernels read lives in one device "arena":
+//   [ItemDesc x items][WorkUnit x units][per-item tables][shared tables]
+// Offsets below are byte offsets into the arena.
+#pragma once
+
+#include <cstdint>
+
+namespace pqp {
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kMaxCols = 64;       // widest row-sum vector a kernel keeps in VGPRs
+constexpr int kMaxDigits = 63;     // Gray digits (rows) per permanent
+constexpr int kDdFold = 64;        // FP64 run length folded into dd accumulators
+
+enum ItemFlags : int {
+  kColsAllOnes = 1,   // every column multiplicity is 1: plain product
+  kTopBinary = 2,     // every top digit has radix 2 (rho = 1): bit-level Gray
+};
+
+// One permanent; read by the kernels with scalar loads (the item index is
+// wave-uniform).
+struct alignas(16) ItemDesc {
+  int64_t b_off;        // B: (1 + L + Dt) rows x mcp complex, row 0 = prepended row,
+                        //    rows 1..L inner digits, rows L+1.. top digits (unscaled)
+  int64_t b2_off;       // inner +-2B: 2L rows x mcp complex (row 2k = +2B_k, 2k+1 = -2B_k)
+  int64_t step_off;     // uint32 per inner step (S_in - 1): row of the b2 table to add
+  int64_t coef_off;     // double per inner state (S_in): (-1)^v prod_inner C(rho, g)
+  int64_t alpha_off;    // double per inner digit (L): rho_k (inner digits start at g = 0)
+  int64_t top_off;      // double x 4 per top digit: {1 / P_t, radix, rho, 0} with place
+                        //    value P_t = prod_{t' < t} radix_t'
+  int64_t cpow_off;     // int32 per r = 2..cmax: number of columns with c_j >= r
+  int64_t S_in;         // inner states per lane
+  int64_t U;            // top states (lanes of work)
+  int64_t partial_base; // first partial slot of this permanent
+  int32_t units;        // waves of work: ceil(U / 64)
+  int32_t mcp;          // compile-time width of the kernel variant (>= mc)
+  int32_t mc;           // columns with non-zero multiplicity (sorted by c_j, descending)
+  int32_t n;            // sum of multiplicities
+  int32_t L;            // inner digits
+  int32_t Dt;           // top digits
+  int32_t flags;        // ItemFlags
+  int32_t cmax;         // largest column multiplicity
+  int32_t out_index;    // result slot
+  int32_t pad0;
+};
+static_assert(sizeof(ItemDesc) == 128, "ItemDesc layout");
+
+// A unit of work: one wavefront sweeps top states [64*local, 64*local + 64)
+// of permanent `item`.
+struct WorkUnit {
+  int32_t item;
+  int32_t local;
+};
+
+// Kernel parameters shared by every launch.
+struct LaunchArgs {
+  const char *arena;
+  const ItemDesc *items;
+  const WorkUnit *units;
+  double *partials;     // 4 doubles (re.hi, re.lo, im.hi, im.lo) per unit
+  double *out;          // 2 doubles per permanent
+  double *out_dd;       // 4 doubles per permanent (may be null)
+  int *queue_head;      // dynamic work-queue head (persistent launches)
+  int32_t n_units;
+  int32_t n_items;
+};
+
+}  // namespace pqp

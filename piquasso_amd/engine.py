@@ -1,0 +1,205 @@
+"""Python face of the permanent engine beyond the drop-in ``permanent()``.
+
+* :func:`permanent_ex` — one permanent with an explicit precision mode and
+  the double-double parts of the result;
+* :class:`PermanentBatch` / :func:`permanent_batch` — many permanents of
+  uneven size in one persistent, work-stealing launch (optionally replayed
+  from a hipGraph).  The reference has no batched entry; every item's value
+  equals ``permanent(matrix, rows, cols)`` on that item.
+
+All arithmetic runs in ``libpqp_permanent.so`` on the GPU.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _capi
+
+__all__ = ["permanent_ex", "PermanentBatch", "permanent_batch", "MODES"]
+
+# precision modes of a single evaluation
+MODES = {
+    "dd": 0,                      # FP64 row sums/products, double-double accumulation
+    "plain": _capi.PQP_FLAG_PLAIN,  # FP64 everywhere, one plain launch (baseline)
+    "fp32": _capi.PQP_FLAG_FP32,    # FP32 row sums/products, dd accumulation
+}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+
+
+def _prep(matrix, rows, cols, dtype=np.complex128):
+    a = np.ascontiguousarray(np.asarray(matrix, dtype=dtype))
+    if a.ndim != 2:
+        raise ValueError(f"matrix must be 2-D, got shape {a.shape}")
+    r = np.ascontiguousarray(np.asarray(rows).astype(np.int32, casting="unsafe").reshape(-1))
+    c = np.ascontiguousarray(np.asarray(cols).astype(np.int32, casting="unsafe").reshape(-1))
+    if r.shape[0] != a.shape[0] or c.shape[0] != a.shape[1]:
+        raise ValueError(
+            f"rows/cols lengths {r.shape[0]}/{c.shape[0]} do not match matrix shape {a.shape}"
+        )
+    return a, r, c
+
+
+def permanent_ex(matrix, rows, cols, mode: str = "dd"):
+    """Return ``(value, dd)`` for one permanent of a complex128 matrix:
+    ``value`` the complex result, ``dd`` = [re_hi, re_lo, im_hi, im_lo]."""
+    if mode not in MODES:
+        raise ValueError(f"mode must be one of {sorted(MODES)}")
+    a, r, c = _prep(matrix, rows, cols)
+    out = np.zeros(2)
+    dd = np.zeros(4)
+    st = _capi.lib().pqp_permanent_c128(
+        a.view(np.float64).ctypes.data_as(_dp), a.shape[0], a.shape[1], a.shape[1],
+        r.ctypes.data_as(_i32p), c.ctypes.data_as(_i32p), MODES[mode],
+        out.ctypes.data_as(_dp), dd.ctypes.data_as(_dp),
+    )
+    _capi.check(st)
+    return complex(out[0], out[1]), dd
+
+
+class PermanentBatch:
+    """A planned batch of permanents resident on the GPU.
+
+    Parameters
+    ----------
+    matrices, rows, cols:
+        sequences of equal length: matrices (complex128, or complex64 which
+        selects FP32 products) and their row / column multiplicities.
+    precision:
+        "fp64" (FP64 row sums and products, double-double accumulation) or
+        "fp32" (FP32 row sums and products, double-double accumulation).
+    persistent:
+        True: one persistent work-stealing launch per column-width bucket;
+        False: one block per work unit.
+    target_inner:
+        inner Gray states per lane (0 lets the planner's cost model choose;
+        the autotuner in :mod:`piquasso_amd.tune` picks it per workload).
+    """
+
+    def __init__(
+        self,
+        matrices: Sequence,
+        rows: Sequence,
+        cols: Sequence,
+        precision: str = "fp64",
+        persistent: bool = True,
+        target_inner: int = 0,
+    ):
+        if not (len(matrices) == len(rows) == len(cols)):
+            raise ValueError("matrices, rows and cols must have the same length")
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' or 'fp32'")
+        self.count = len(matrices)
+        self.precision = precision
+        self._keep = []
+        mats, rws, cls = [], [], []
+        for m, r, c in zip(matrices, rows, cols):
+            a, rr, cc = _prep(m, r, c)
+            self._keep.extend((a, rr, cc))
+            mats.append(a.ctypes.data)
+            rws.append(rr.ctypes.data_as(_i32p))
+            cls.append(cc.ctypes.data_as(_i32p))
+        n = self.count
+        mat_arr = (ctypes.c_void_p * max(n, 1))(*mats)
+        row_arr = (_i32p * max(n, 1))(*rws)
+        col_arr = (_i32p * max(n, 1))(*cls)
+        m_r = np.array([self._keep[3 * k].shape[0] for k in range(n)] or [0], dtype=np.int64)
+        m_c = np.array([self._keep[3 * k].shape[1] for k in range(n)] or [0], dtype=np.int64)
+        flags = 0
+        if precision == "fp32":
+            flags |= _capi.PQP_FLAG_FP32
+        if not persistent:
+            flags |= _capi.PQP_FLAG_STATIC
+        handle = ctypes.c_void_p()
+        L = _capi.lib()
+        st = L.pqp_batch_create(
+            n, mat_arr, 0, m_r.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+            m_c.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+            m_c.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+            row_arr, col_arr, flags, int(target_inner), ctypes.byref(handle),
+        )
+        _capi.check(st)
+        self._h = handle
+        self._keep = None  # inputs were copied into the plan
+
+    # -- running ---------------------------------------------------------
+    def run(self, mode: str = "graph") -> "PermanentBatch":
+        """Enqueue one evaluation: mode "graph" (hipGraph replay), "eager"
+        (persistent launches) or "plain" (plain FP64 kernel, one launch per
+        permanent)."""
+        code = {"eager": _capi.PQP_RUN_EAGER, "graph": _capi.PQP_RUN_GRAPH,
+                "plain": _capi.PQP_RUN_PLAIN}[mode]
+        _capi.check(_capi.lib().pqp_batch_run(self._h, code))
+        return self
+
+    def sync(self) -> None:
+        _capi.check(_capi.lib().pqp_batch_sync(self._h))
+
+    def results(self, with_dd: bool = False):
+        """Results of the last run as complex128 (and the dd parts)."""
+        out = np.zeros((max(self.count, 1), 2))
+        dd = np.zeros((max(self.count, 1), 4))
+        _capi.check(
+            _capi.lib().pqp_batch_results(
+                self._h, out.ctypes.data_as(_dp), dd.ctypes.data_as(_dp) if with_dd else None
+            )
+        )
+        vals = (out[:, 0] + 1j * out[:, 1])[: self.count]
+        return (vals, dd[: self.count]) if with_dd else vals
+
+    def time(self, mode: str = "graph", reps: int = 5) -> np.ndarray:
+        """Milliseconds per evaluation for `reps` evaluations, timed with HIP
+        events on the batch stream.  mode "kernel" times the sweep kernels
+        alone (the finalize kernel runs outside the events)."""
+        code = {"eager": _capi.PQP_RUN_EAGER, "graph": _capi.PQP_RUN_GRAPH,
+                "plain": _capi.PQP_RUN_PLAIN, "kernel": _capi.PQP_RUN_KERNEL}[mode]
+        ms = np.zeros(max(reps, 1), dtype=np.float32)
+        _capi.check(
+            _capi.lib().pqp_batch_time(self._h, code, reps, ms.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        )
+        return ms[:reps].astype(np.float64)
+
+    def stats(self) -> dict:
+        flops = ctypes.c_double(0)
+        terms = ctypes.c_int64(0)
+        units = ctypes.c_int64(0)
+        arena = ctypes.c_int64(0)
+        _capi.check(
+            _capi.lib().pqp_batch_stats(
+                self._h, ctypes.byref(flops), ctypes.byref(terms), ctypes.byref(units),
+                ctypes.byref(arena),
+            )
+        )
+        return {"flops": flops.value, "terms": terms.value, "units": units.value,
+                "arena_bytes": arena.value}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _capi.lib().pqp_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def permanent_batch(matrices, rows, cols, precision: str = "fp64",
+                    target_inner: int = 0) -> np.ndarray:
+    """Permanents of many (matrix, rows, cols) triples in one launch."""
+    with PermanentBatch(matrices, rows, cols, precision=precision,
+                        target_inner=target_inner) as b:
+        b.run("eager")
+        return b.results()

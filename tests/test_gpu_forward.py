@@ -1,0 +1,204 @@
+"""Parity of the HIP forward kernels with the oracle (GPU).
+
+Every value is compared with the double-double oracle (oracle/glynn_dd.c) at
+1e-12 relative error for sizes the oracle finishes quickly, bit-exactly for
+integer matrices whose terms stay below 2^53, and with the reference's own
+known-answer vectors.  The 1e-10 bar of the engine's specification applies
+to the large cases in test_gpu_large.py.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+import piquasso_amd as pa
+from conftest import complex_matrix, rel_err
+from piquasso_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _case_matrix(case):
+    if "shape" in case:
+        return complex_matrix(case["matrix"]).reshape(case["shape"])
+    return complex_matrix(case["matrix"])
+
+
+def test_reference_vectors(gpu, oracle, reference_vectors):
+    for case in reference_vectors["permanent"]:
+        a = _case_matrix(case)
+        r, c = case["rows"], case["cols"]
+        got = pa.permanent(a, r, c)
+        assert got.dtype == np.complex128 and got.shape == ()
+        check = case.get("check")
+        if check == "conjugate_symmetry":
+            assert rel_err(pa.permanent(np.conj(a), r, c), np.conj(got)) < TOL
+            continue
+        ref = oracle.perm_dd(a, r, c)
+        if case.get("exact"):
+            assert complex(got) == ref, case["id"]
+        else:
+            assert abs(complex(got) - ref) <= TOL * max(abs(ref), 1e-300), (case["id"], got, ref)
+
+
+def test_reference_probabilities(gpu, reference_vectors):
+    for case in reference_vectors["probability"]:
+        u = _case_matrix(case)
+        inp, out = case["input"], case["output"]
+        amp = complex(pa.permanent(u, cols=inp, rows=out))
+        norm = np.prod([math.factorial(k) for k in out]) * np.prod([math.factorial(k) for k in inp])
+        assert abs(abs(amp) ** 2 / norm - case["expected"]) < 1e-8, case["id"]
+
+
+def test_real_input_dispatch_matches_reference(gpu):
+    got = pa.permanent(np.array([[4.2]]), np.ones(1, int), np.ones(1, int))
+    assert got.dtype == np.complex128 and complex(got) == 4.2
+    got = pa.permanent([[1.5]], [1], [1])
+    assert got.dtype == np.complex64 and complex(got) == 1.5
+
+
+def test_random_with_multiplicities(gpu, oracle):
+    rng = np.random.default_rng(100)
+    for trial in range(40):
+        m_r, m_c = (int(x) for x in rng.integers(1, 10, size=2))
+        n = int(rng.integers(0, 15))
+        a = (rng.standard_normal((m_r, m_c)) + 1j * rng.standard_normal((m_r, m_c)))
+        r = np.bincount(rng.integers(0, m_r, n), minlength=m_r)
+        c = np.bincount(rng.integers(0, m_c, n), minlength=m_c)
+        got = complex(pa.permanent(a, r, c))
+        ref = oracle.perm_dd(a, r, c)
+        assert rel_err(got, ref) < TOL, (trial, m_r, m_c, r, c, got, ref)
+
+
+def test_every_kernel_width(gpu, oracle):
+    """m_c = 1..64 covers every exact-width kernel and the padded ones (>32)."""
+    rng = np.random.default_rng(7)
+    mats, rows, cols, refs = [], [], [], []
+    for mc in range(1, 65):
+        # few Gray terms: two rows carrying all the multiplicity
+        a = (rng.standard_normal((2, mc)) + 1j * rng.standard_normal((2, mc))) / np.sqrt(mc)
+        r = np.array([mc // 2, mc - mc // 2])
+        c = np.ones(mc, int)
+        mats.append(a)
+        rows.append(r)
+        cols.append(c)
+        refs.append(oracle.perm_dd(a, r, c))
+    for k in (0, 7, 8, 15, 23, 31, 32, 39, 47, 55, 63):
+        got = complex(pa.permanent(mats[k], rows[k], cols[k]))
+        assert rel_err(got, refs[k]) < TOL, k + 1
+    vals = pa.permanent_batch(mats, rows, cols)
+    for k in range(64):
+        assert rel_err(vals[k], refs[k]) < TOL, k + 1
+
+
+def test_integer_matrices_exact(gpu):
+    for n in range(1, 14):
+        ones = np.ones(n, int)
+        J = np.ones((n, n))
+        assert complex(pa.permanent(J.astype(complex), ones, ones)) == math.factorial(n)
+        der = round(math.factorial(n) * sum((-1) ** k / math.factorial(k) for k in range(n + 1)))
+        assert complex(pa.permanent((J - np.eye(n)).astype(complex), ones, ones)) == der
+    rng = np.random.default_rng(3)
+    for trial in range(10):
+        n = int(rng.integers(2, 12))
+        a = rng.integers(0, 2, size=(n, n))
+        ones = np.ones(n, int)
+        exact = _perm_int_ryser(a)
+        assert complex(pa.permanent(a.astype(complex), ones, ones)) == exact
+
+
+def _perm_int_ryser(a):
+    n = a.shape[0]
+    total = 0
+    for mask in range(1, 1 << n):
+        cols = [j for j in range(n) if mask >> j & 1]
+        prod = 1
+        for i in range(n):
+            prod *= int(sum(int(a[i, j]) for j in cols))
+        total += (-1) ** (n - len(cols)) * prod
+    return total
+
+
+def test_modes_agree(gpu, oracle):
+    rng = np.random.default_rng(5)
+    for n in (6, 11, 17):
+        a = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(2 * n)
+        ones = np.ones(n, int)
+        ref = oracle.perm_dd(a, ones, ones)
+        v_dd, dd = pa.permanent_ex(a, ones, ones, mode="dd")
+        v_plain, _ = pa.permanent_ex(a, ones, ones, mode="plain")
+        v_32, _ = pa.permanent_ex(a, ones, ones, mode="fp32")
+        assert rel_err(v_dd, ref) < TOL
+        assert rel_err(complex(dd[0] + dd[1], dd[2] + dd[3]), ref) < TOL
+        assert rel_err(v_plain, ref) < 1e-10
+        assert rel_err(v_32, ref) < 5e-3
+
+
+def test_complex64_path(gpu, oracle):
+    rng = np.random.default_rng(8)
+    a = ((rng.standard_normal((9, 9)) + 1j * rng.standard_normal((9, 9))) / 3).astype(np.complex64)
+    r = np.array([1, 1, 2, 0, 1, 1, 1, 1, 1])
+    c = np.array([1, 1, 1, 1, 1, 2, 0, 1, 1])
+    got = pa.permanent(a, r, c)
+    assert got.dtype == np.complex64 and got.shape == ()
+    ref = oracle.perm_dd(a.astype(np.complex128), r, c)
+    assert rel_err(complex(got), ref) < 1e-4
+
+
+def test_errors(gpu):
+    with pytest.raises(RuntimeError):
+        pa.permanent(np.eye(3, dtype=complex), [1, 1, 1], [1, 1, 0])
+    with pytest.raises(ValueError):
+        pa.permanent(np.eye(3, dtype=complex), [1, 1], [1, 1, 1])
+    with pytest.raises(ValueError):
+        pa.permanent(np.eye(2, dtype=complex), [2, -1], [1, 0])
+    assert complex(pa.permanent(np.zeros((0, 0), dtype=complex), [], [])) == 1.0
+    assert complex(pa.permanent(np.eye(3, dtype=complex), [0, 0, 0], [0, 0, 0])) == 1.0
+
+
+def test_batch_matches_single_and_oracle(gpu, oracle):
+    mats, rows, cols = W.batched_uneven(count=96, seed=44, n_lo=4, n_hi=14)
+    refs = [oracle.perm_dd(m, r, c) for m, r, c in zip(mats, rows, cols)]
+    with pa.PermanentBatch(mats, rows, cols) as b:
+        eager = b.run("eager").results()
+        graph = [b.run("graph").results() for _ in range(3)]
+        plain = b.run("plain").results()
+        eager2 = b.run("eager").results()
+        graph2 = b.run("graph").results()
+    for k in range(len(mats)):
+        assert rel_err(eager[k], refs[k]) < TOL, k
+        assert rel_err(plain[k], refs[k]) < 1e-10, k
+    # deterministic: replays and eager runs give identical bits
+    for g in graph + [graph2, eager2]:
+        assert np.array_equal(g, eager)
+    single = [complex(pa.permanent(m, r, c)) for m, r, c in zip(mats[:8], rows[:8], cols[:8])]
+    for k in range(8):
+        assert rel_err(single[k], refs[k]) < TOL
+
+
+def test_static_and_persistent_agree(gpu):
+    mats, rows, cols = W.batched_uneven(count=64, seed=45, n_lo=8, n_hi=16)
+    with pa.PermanentBatch(mats, rows, cols, persistent=True) as b1:
+        v1 = b1.run("eager").results()
+    with pa.PermanentBatch(mats, rows, cols, persistent=False) as b2:
+        v2 = b2.run("eager").results()
+    assert np.array_equal(v1, v2)
+
+
+def test_inner_size_does_not_change_the_value(gpu, oracle):
+    a, r, c = W.large_single(n=16, seed=9)
+    ref = oracle.perm_dd(a, r, c)
+    for target in (1, 2, 8, 64, 1024, 1 << 15):
+        with pa.PermanentBatch([a], [r], [c], target_inner=target) as b:
+            got = b.run("eager").results()[0]
+        assert rel_err(got, ref) < TOL, target
+
+
+def test_boson_sampling_probability(gpu, oracle):
+    u, rows, cols = W.boson_sampling(seed=2)
+    got = complex(pa.permanent(u, rows, cols))
+    ref = oracle.perm_dd(u, rows, cols)
+    assert rel_err(got, ref) < 1e-11

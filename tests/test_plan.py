@@ -1,0 +1,141 @@
+"""Host logic (CPU only): the planner's device arena, checked by emulating the
+kernels' arithmetic on it in numpy and comparing with the oracle.
+
+pqp_plan_image returns the exact bytes the HIP kernels read.  The emulation
+below follows perm_fwd.hip step by step (start row sums per lane from the
+B rows and the lane's top Gray digits, inner sweep adding the tabulated
++-2B rows, coefficient table, lane weight and sign, 2^-(n-1)), so a wrong
+table, offset, sign, weight or digit split fails here without a GPU."""
+
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+from piquasso_amd import _capi
+
+ITEM = np.dtype([
+    ("b_off", "<i8"), ("b2_off", "<i8"), ("step_off", "<i8"), ("coef_off", "<i8"),
+    ("alpha_off", "<i8"), ("top_off", "<i8"), ("cpow_off", "<i8"), ("S_in", "<i8"),
+    ("U", "<i8"), ("partial_base", "<i8"), ("units", "<i4"), ("mcp", "<i4"), ("mc", "<i4"),
+    ("n", "<i4"), ("L", "<i4"), ("Dt", "<i4"), ("flags", "<i4"), ("cmax", "<i4"),
+    ("out_index", "<i4"), ("pad0", "<i4"), ("tail", "V8"),
+])
+assert ITEM.itemsize == 128
+
+
+def plan_image(a, rows, cols, target_inner=0, min_units=0):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
+    r = np.ascontiguousarray(np.asarray(rows, dtype=np.int32))
+    c = np.ascontiguousarray(np.asarray(cols, dtype=np.int32))
+    L = _capi.lib()
+    need = ctypes.c_int64(0)
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int32)
+    args = (a.view(np.float64).ctypes.data_as(dp), a.shape[0], a.shape[1], a.shape[1],
+            r.ctypes.data_as(ip), c.ctypes.data_as(ip), 0, target_inner, min_units)
+    _capi.check(L.pqp_plan_image(*args, None, 0, ctypes.byref(need)))
+    buf = np.zeros(need.value, dtype=np.uint8)
+    _capi.check(L.pqp_plan_image(*args, buf.ctypes.data, need.value, ctypes.byref(need)))
+    return buf
+
+
+def emulate(buf):
+    """The kernels' arithmetic on the planned arena, in numpy (complex128)."""
+    d = np.frombuffer(buf[:128].tobytes(), dtype=ITEM)[0]
+    if d["n"] == 0:
+        return 1.0 + 0j
+    mcp, L, Dt, S, U = int(d["mcp"]), int(d["L"]), int(d["Dt"]), int(d["S_in"]), int(d["U"])
+
+    def f64(off, count):
+        return np.frombuffer(buf[off: off + 8 * count].tobytes(), dtype=np.float64)
+
+    B = f64(int(d["b_off"]), (1 + L + Dt) * mcp * 2).view(np.complex128).reshape(1 + L + Dt, mcp)
+    B2 = f64(int(d["b2_off"]), 2 * max(L, 1) * mcp * 2).view(np.complex128).reshape(-1, mcp)
+    alpha = f64(int(d["alpha_off"]), max(L, 1))
+    coef = f64(int(d["coef_off"]), S)
+    steps = np.frombuffer(buf[int(d["step_off"]): int(d["step_off"]) + 4 * max(S - 1, 0)].tobytes(),
+                          dtype=np.uint32)
+    top = f64(int(d["top_off"]), 4 * max(Dt, 1)).reshape(-1, 4)
+    cmax = int(d["cmax"])
+    cpow = np.frombuffer(buf[int(d["cpow_off"]): int(d["cpow_off"]) + 4 * max(cmax - 1, 1)].tobytes(),
+                         dtype=np.int32)
+    total = 0j
+    for u in range(U):
+        cs = B[0].copy()
+        for k in range(L):
+            cs = cs + alpha[k] * B[1 + k]
+        w = 1.0
+        parity = 0
+        q_hi = 0.0
+        for t in range(Dt - 1, -1, -1):
+            inv_place, radix, rho = top[t, 0], top[t, 1], int(top[t, 2])
+            q = math.floor((u + 0.5) * inv_place)
+            cdig = int(q - radix * q_hi)
+            q_hi = q
+            g = int(radix) - 1 - cdig if parity else cdig
+            parity ^= g & 1
+            w *= math.comb(rho, g)
+            cs = cs + (rho - 2 * g) * B[1 + L + t]
+        lane_w = -w if (u & 1) else w
+        acc = 0j
+        for v in range(S):
+            p = np.prod(cs)
+            for rr in range(2, cmax + 1):
+                p *= np.prod(cs[: cpow[rr - 2]])
+            acc += coef[v] * p
+            if v + 1 < S:
+                cs = cs + B2[steps[v]]
+        total += lane_w * acc
+    return total / 2.0 ** (int(d["n"]) - 1)
+
+
+def test_plan_emulation_matches_oracle(oracle):
+    rng = np.random.default_rng(31)
+    for trial in range(40):
+        m_r, m_c = (int(x) for x in rng.integers(1, 7, size=2))
+        n = int(rng.integers(0, 9))
+        a = rng.standard_normal((m_r, m_c)) + 1j * rng.standard_normal((m_r, m_c))
+        r = np.bincount(rng.integers(0, m_r, n), minlength=m_r)
+        c = np.bincount(rng.integers(0, m_c, n), minlength=m_c)
+        ref = oracle.perm_dd(a, r, c)
+        for target in (0, 1, 2, 3, 8, 1 << 20):
+            got = emulate(plan_image(a, r, c, target_inner=target))
+            assert abs(got - ref) <= 1e-11 * max(abs(ref), 1e-300), (trial, target, got, ref)
+
+
+def test_plan_splits_large_items(oracle):
+    # 2^12 terms: min_units forces many lanes with short inner sweeps
+    rng = np.random.default_rng(2)
+    n = 13
+    a = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n)
+    ones = np.ones(n, int)
+    buf = plan_image(a, ones, ones, min_units=4)
+    d = np.frombuffer(buf[:128].tobytes(), dtype=ITEM)[0]
+    assert d["units"] >= 4 and d["S_in"] * d["U"] == 2 ** (n - 1)
+    assert d["flags"] & 2  # all top digits binary
+    ref = oracle.perm_dd(a, ones, ones)
+    assert abs(emulate(buf) - ref) <= 1e-11 * abs(ref)
+
+
+def test_padding_columns_are_neutral(oracle):
+    rng = np.random.default_rng(4)
+    mc = 37  # padded to a 40-wide kernel
+    a = (rng.standard_normal((2, mc)) + 1j * rng.standard_normal((2, mc))) / np.sqrt(mc)
+    r = np.array([18, 19])
+    c = np.ones(mc, int)
+    buf = plan_image(a, r, c)
+    d = np.frombuffer(buf[:128].tobytes(), dtype=ITEM)[0]
+    assert d["mcp"] == 40 and d["mc"] == 37
+    ref = oracle.perm_dd(a, r, c)
+    assert abs(emulate(buf) - ref) <= 1e-9 * abs(ref)
+
+
+def test_plan_errors():
+    with pytest.raises(RuntimeError):
+        plan_image(np.eye(2), [1, 1], [2, 1])
+    with pytest.raises(ValueError):
+        plan_image(np.eye(2), [1, -1], [0, 0])
+    with pytest.raises(ValueError):
+        plan_image(np.ones((1, 65)), [65], np.ones(65))
