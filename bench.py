@@ -62,7 +62,8 @@ def _load_n32():
 def cpu_baseline_child(seconds: float) -> None:
     """Runs in a child process (before the parent touches the GPU): time the
     reference's own permanent (oracle/_ref, built from its sources) on a
-    prefix of the batch until `seconds` of wall time are used."""
+    the batch, cycling over its items in order, until `seconds` of wall time
+    are used."""
     from oracle import oracle as O
     from piquasso_amd import workloads as W
 
@@ -76,9 +77,10 @@ def cpu_baseline_child(seconds: float) -> None:
     done = 0
     flops = 0.0
     t0 = time.perf_counter()
-    while done < len(mats) and time.perf_counter() - t0 < seconds:
-        fn(mats[done], rows[done], cols[done])
-        flops += W.glynn_flops(rows[done], cols[done])
+    while time.perf_counter() - t0 < seconds:
+        k = done % len(mats)
+        fn(mats[k], rows[k], cols[k])
+        flops += W.glynn_flops(rows[k], cols[k])
         done += 1
     dt = time.perf_counter() - t0
     print(json.dumps({"kind": kind, "items": done, "seconds": dt, "flops": flops,
@@ -108,8 +110,9 @@ def run_cpu_baseline(seconds: float):
         "unit": "permanents/s",
         "cores": threads,
         "kind": rec["kind"],
-        "sample": (f"first {rec['items']} of the 4096 batched permanents "
-                   f"({rec['flops']:.3e} algorithmic FLOP) in {rec['seconds']:.1f} s, "
+        "sample": (f"{rec['items']} permanents of the batched workload, in order, "
+                   f"cycling over its 4096 items ({rec['flops']:.3e} algorithmic FLOP) in "
+                   f"{rec['seconds']:.1f} s, reference pybind permanent(), "
                    f"OMP_THREAD_LIMIT={threads}"),
         "gflops": rec["flops"] / rec["seconds"] / 1e9 if rec["seconds"] > 0 else None,
     }
@@ -125,10 +128,38 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-child", type=float, default=None)
+    ap.add_argument("--profile", choices=["batch", "n32"], default=None,
+                    help="profiling runs: only launch this workload's kernels --steps times")
+    ap.add_argument("--copy-calibration", type=int, default=None, metavar="MIB",
+                    help="profiling runs: one device-to-device copy of MIB MiB (torch), to "
+                         "calibrate FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
 
     if args.cpu_child is not None:
         cpu_baseline_child(args.cpu_child)
+        return 0
+    if args.copy_calibration is not None:
+        import torch
+
+        n = args.copy_calibration * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": 4 * n}))
+        return 0
+    if args.profile is not None:
+        import piquasso_amd as pa
+
+        if args.profile == "batch":
+            mats, rows, cols, _ = _load_batched()
+            b = pa.PermanentBatch(mats, rows, cols)
+        else:
+            a32, r32, c32, _ = _load_n32()
+            b = pa.PermanentBatch([a32], [r32], [c32], persistent=False)
+        ms = b.time("eager", reps=args.steps)
+        print(json.dumps({"profile": args.profile, "ms": ms.tolist()}))
         return 0
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "

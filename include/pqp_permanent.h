@@ -46,6 +46,11 @@ extern "C" {
 #define PQP_FLAG_PLAIN 0x1u  /* plain FP64 accumulation and one launch per permanent (the baseline) */
 #define PQP_FLAG_FP32 0x2u   /* FP32 row sums and products, FP64 + double-double accumulation */
 #define PQP_FLAG_STATIC 0x4u /* batches: one block per work unit instead of the persistent queue */
+/* Columns with multiplicity c > 1 are, by default, kept as one column whose
+ * row sum is raised to the power c; PQP_FLAG_EXPAND_COLS expands them into c
+ * identical columns instead (branch-free product, wider kernel). */
+#define PQP_FLAG_EXPAND_COLS 0x8u
+#define PQP_FLAG_KEEP_COLS 0x10u
 
 /* Library version string. */
 const char *pqp_version(void);
@@ -94,13 +99,15 @@ int pqp_permanent_c64(const float *matrix, int64_t m_r, int64_t m_c, int64_t ld,
  *   m_r, m_c, ld : count entries each
  *   rows, cols   : count pointers to int32 multiplicities
  *   target_inner : inner Gray states per lane (0 = the planner's cost model)
+ *   total_units  : wave work units to spread over the batch in proportion to
+ *                  each permanent's cost (0 = 48 per compute unit)
  */
 typedef struct pqp_batch pqp_batch;
 
 int pqp_batch_create(int64_t count, const void *const *matrices, int is_c64, const int64_t *m_r,
                      const int64_t *m_c, const int64_t *ld, const int32_t *const *rows,
                      const int32_t *const *cols, uint32_t flags, int64_t target_inner,
-                     pqp_batch **out);
+                     int64_t total_units, pqp_batch **out);
 
 /* Run modes for pqp_batch_run / pqp_batch_time */
 #define PQP_RUN_EAGER 0  /* persistent kernel(s) + finalize, launched on the batch stream */

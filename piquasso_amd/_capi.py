@@ -27,6 +27,8 @@ PQP_ERR_NOKERNEL = -6
 PQP_FLAG_PLAIN = 0x1
 PQP_FLAG_FP32 = 0x2
 PQP_FLAG_STATIC = 0x4
+PQP_FLAG_EXPAND_COLS = 0x8
+PQP_FLAG_KEEP_COLS = 0x10
 
 PQP_RUN_EAGER = 0
 PQP_RUN_GRAPH = 1
@@ -57,7 +59,7 @@ SIGNATURES = {
         _c.c_int,
         [
             _c.c_int64, _c.POINTER(_vp), _c.c_int, _i64p, _i64p, _i64p,
-            _c.POINTER(_i32p), _c.POINTER(_i32p), _c.c_uint32, _c.c_int64,
+            _c.POINTER(_i32p), _c.POINTER(_i32p), _c.c_uint32, _c.c_int64, _c.c_int64,
             _c.POINTER(_vp),
         ],
     ),

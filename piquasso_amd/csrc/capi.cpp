@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -48,7 +49,7 @@ struct DeviceState {
   std::string name;
   std::string error;
   // occupancy cache: [bucket][fp32][comp][persistent], -1 = not queried
-  int occ[8][2][2][2];
+  int occ[kBuckets][2][2][2];
 };
 
 DeviceState &device_state() {
@@ -80,6 +81,12 @@ DeviceState &device_state() {
 // Work units per CU a launch is split into: about 3 resident waves per SIMD
 // (12 per CU) plus slack for load balance.
 constexpr int64_t kUnitsPerCu = 48;
+
+int expand_mode(uint32_t flags) {
+  if (flags & PQP_FLAG_EXPAND_COLS) return 1;
+  if (flags & PQP_FLAG_KEEP_COLS) return 0;
+  return 2;
+}
 
 int require_device() {
   DeviceState &d = device_state();
@@ -131,6 +138,10 @@ struct Exec {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   uint32_t flags = 0;
+  // side streams and events for running several width buckets concurrently
+  hipStream_t side[kBuckets] = {};
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join[kBuckets] = {};
 
   LaunchArgs args() const {
     LaunchArgs a;
@@ -170,6 +181,17 @@ int upload(Exec &x) {
   return PQP_OK;
 }
 
+// Side streams (default priority) and fork/join events, created once per Exec.
+int ensure_side_streams(Exec &x) {
+  if (x.ev_fork) return PQP_OK;
+  for (int i = 1; i < kBuckets; ++i) {
+    PQP_HIP(hipStreamCreateWithFlags(&x.side[i], hipStreamNonBlocking));
+    PQP_HIP(hipEventCreateWithFlags(&x.ev_join[i], hipEventDisableTiming));
+  }
+  PQP_HIP(hipEventCreateWithFlags(&x.ev_fork, hipEventDisableTiming));
+  return PQP_OK;
+}
+
 // Enqueue one evaluation.  ev_begin / ev_end (optional) bracket the sweep
 // kernels only when kernel_only is set, the whole evaluation otherwise.
 int enqueue(Exec &x, int mode, hipEvent_t ev_begin, hipEvent_t ev_end) {
@@ -199,22 +221,48 @@ int enqueue(Exec &x, int mode, hipEvent_t ev_begin, hipEvent_t ev_end) {
   } else {
     const bool fixed = (x.flags & PQP_FLAG_STATIC) != 0;
     const DeviceState &dev = device_state();
-    int slot = 0;
-    for (const Plan::Range &r : p.ranges) {
+    // Width buckets run concurrently: the costliest range launches first on
+    // the main stream, the others on side streams forked from it, so their
+    // waves fill the slots the first one's tail leaves idle.
+    std::vector<int> order(p.ranges.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = static_cast<int>(i);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int u, int v) { return p.ranges[u].cost > p.ranges[v].cost; });
+    const bool fork = order.size() > 1;
+    double total_cost = 0.0;
+    for (const Plan::Range &r : p.ranges) total_cost += r.cost;
+    if (fork) {
+      int st = ensure_side_streams(x);
+      if (st != PQP_OK) return st;
+      PQP_HIP(hipEventRecord(x.ev_fork, x.stream));
+    }
+    for (size_t i = 0; i < order.size(); ++i) {
+      const int ri = order[i];
+      const Plan::Range &r = p.ranges[ri];
+      hipStream_t s = (i == 0) ? x.stream : x.side[i];
+      if (i > 0) PQP_HIP(hipStreamWaitEvent(s, x.ev_fork, 0));
       FwdLaunch l;
       l.args = a;
       l.first_unit = r.first;
       l.n_units = r.count;
-      l.head_slot = slot++;
+      l.head_slot = ri;
       l.persistent = !fixed;
       l.fp32 = p.fp32;
       l.comp = !(x.flags & PQP_FLAG_PLAIN) || p.fp32;
-      l.stream = x.stream;
+      l.stream = s;
       if (l.persistent) {
+        // resident waves in proportion to the range's share of the work, so
+        // concurrent ranges run side by side instead of one after another
         const int per_cu = blocks_per_cu(r.bucket, l.fp32, l.comp, true);
-        l.grid = std::max(1, std::min(r.count, dev.cus * per_cu));
+        const double share = total_cost > 0.0 ? r.cost / total_cost : 1.0;
+        const int slots = static_cast<int>(std::ceil(share * dev.cus * per_cu));
+        l.grid = std::max(1, std::min(r.count, std::max(slots, dev.cus / 4)));
       }
       PQP_HIP(launch_fwd(r.bucket, l));
+      if (i > 0) {
+        PQP_HIP(hipEventRecord(x.ev_join[i], s));
+        PQP_HIP(hipStreamWaitEvent(x.stream, x.ev_join[i], 0));
+      }
     }
     if (kernel_only && ev_end) PQP_HIP(hipEventRecord(ev_end, x.stream));
     const bool comp = !(x.flags & PQP_FLAG_PLAIN) || p.fp32;
@@ -253,6 +301,7 @@ int run_single(const PermInput &in, uint32_t flags, double *out2, double *out_dd
   // or without a device.
   PlanOptions opt;
   opt.fp32_products = (flags & PQP_FLAG_FP32) != 0;
+  opt.expand_cols = expand_mode(flags);
   const DeviceState &dev = device_state();
   // enough waves to fill every CU several times over for large permanents
   opt.min_units = static_cast<int64_t>(std::max(dev.cus, 1)) * kUnitsPerCu;
@@ -311,17 +360,17 @@ int pqp_permanent_c128(const double *matrix, int64_t m_r, int64_t m_c, int64_t l
                        double *out_dd) {
   if (!out) return fail(PQP_ERR_INVALID, "out is null");
   PermInput in{matrix, false, m_r, m_c, ld, rows, cols};
-  return run_single(in, flags & (PQP_FLAG_PLAIN | PQP_FLAG_FP32), out, out_dd);
+  return run_single(in, flags, out, out_dd);
 }
 
 int pqp_permanent_c64(const float *matrix, int64_t m_r, int64_t m_c, int64_t ld,
                       const int32_t *rows, const int32_t *cols, uint32_t flags, float *out,
                       double *out_dd) {
   if (!out) return fail(PQP_ERR_INVALID, "out is null");
-  (void)flags;
   PermInput in{matrix, true, m_r, m_c, ld, rows, cols};
   double o[2];
-  const int st = run_single(in, PQP_FLAG_FP32, o, out_dd);
+  const int st =
+      run_single(in, PQP_FLAG_FP32 | (flags & (PQP_FLAG_EXPAND_COLS | PQP_FLAG_KEEP_COLS)), o, out_dd);
   if (st != PQP_OK) return st;
   out[0] = static_cast<float>(o[0]);
   out[1] = static_cast<float>(o[1]);
@@ -331,7 +380,7 @@ int pqp_permanent_c64(const float *matrix, int64_t m_r, int64_t m_c, int64_t ld,
 int pqp_batch_create(int64_t count, const void *const *matrices, int is_c64, const int64_t *m_r,
                      const int64_t *m_c, const int64_t *ld, const int32_t *const *rows,
                      const int32_t *const *cols, uint32_t flags, int64_t target_inner,
-                     pqp_batch **out) {
+                     int64_t total_units, pqp_batch **out) {
   if (!out) return fail(PQP_ERR_INVALID, "out is null");
   *out = nullptr;
   if (count < 0 || (count > 0 && (!matrices || !m_r || !m_c || !ld || !rows || !cols)))
@@ -344,9 +393,12 @@ int pqp_batch_create(int64_t count, const void *const *matrices, int is_c64, con
   if (!b) return fail(PQP_ERR_NOMEM, "out of host memory");
   PlanOptions opt;
   opt.fp32_products = (flags & PQP_FLAG_FP32) != 0 || is_c64 != 0;
+  opt.expand_cols = expand_mode(flags);
   opt.target_inner = target_inner;
   // about kUnitsPerCu waves per CU over the whole batch, split by cost
-  opt.total_units = static_cast<int64_t>(std::max(device_state().cus, 1)) * kUnitsPerCu;
+  opt.total_units = total_units > 0
+                        ? total_units
+                        : static_cast<int64_t>(std::max(device_state().cus, 1)) * kUnitsPerCu;
   std::string err;
   int st = plan_build(inputs, opt, b->x.plan, err);
   if (st != PQP_OK) return fail(st, err);
@@ -456,6 +508,7 @@ int pqp_plan_image(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
   PermInput in{matrix, false, m_r, m_c, ld, rows, cols};
   PlanOptions opt;
   opt.fp32_products = (flags & PQP_FLAG_FP32) != 0;
+  opt.expand_cols = expand_mode(flags);
   opt.target_inner = target_inner;
   opt.min_units = min_units;
   Plan p;
@@ -480,6 +533,11 @@ void pqp_batch_destroy(pqp_batch *b) {
   b->x.out.release();
   b->x.out_dd.release();
   b->x.heads.release();
+  for (int i = 1; i < kBuckets; ++i) {
+    if (b->x.side[i]) (void)hipStreamDestroy(b->x.side[i]);
+    if (b->x.ev_join[i]) (void)hipEventDestroy(b->x.ev_join[i]);
+  }
+  if (b->x.ev_fork) (void)hipEventDestroy(b->x.ev_fork);
   if (b->x.own_stream && b->x.stream) (void)hipStreamDestroy(b->x.stream);
   delete b;
 }

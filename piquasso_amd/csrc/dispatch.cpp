@@ -1,0 +1,62 @@
+// Routes kernel launches and occupancy queries to the translation unit that
+// instantiates the requested (column-width bucket, arithmetic variant).
+#include "perm_kernels.hpp"
+
+namespace pqp {
+
+#define PQP_CASE(k)                                        \
+  case k:                                                  \
+    if (l.fp32) return launch_fwd_b##k##_f(l);             \
+    return l.comp ? launch_fwd_b##k##_d(l) : launch_fwd_b##k##_p(l);
+
+hipError_t launch_fwd(int bucket, const FwdLaunch &l) {
+  if (l.fp32 && !l.comp) return hipErrorInvalidValue;  // FP32 products always use dd
+  switch (bucket) {
+    PQP_CASE(0)
+    PQP_CASE(1)
+    PQP_CASE(2)
+    PQP_CASE(3)
+    PQP_CASE(4)
+    PQP_CASE(5)
+    PQP_CASE(6)
+    PQP_CASE(7)
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+#undef PQP_CASE
+
+#define PQP_CASE(k)                                                        \
+  case k:                                                                  \
+    if (fp32) return occupancy_fwd_b##k##_f(persistent, blocks);           \
+    return comp ? occupancy_fwd_b##k##_d(persistent, blocks)               \
+                : occupancy_fwd_b##k##_p(persistent, blocks);
+
+hipError_t occupancy_fwd(int bucket, bool persistent, bool fp32, bool comp, int *blocks) {
+  switch (bucket) {
+    PQP_CASE(0)
+    PQP_CASE(1)
+    PQP_CASE(2)
+    PQP_CASE(3)
+    PQP_CASE(4)
+    PQP_CASE(5)
+    PQP_CASE(6)
+    PQP_CASE(7)
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+#undef PQP_CASE
+
+hipError_t launch_minors(int bucket, const LaunchArgs &a, int first_unit, int n_units,
+                         hipStream_t stream) {
+  switch (bucket) {
+    case 0: return launch_minors_b0(a, first_unit, n_units, stream);
+    case 1: return launch_minors_b1(a, first_unit, n_units, stream);
+    case 2: return launch_minors_b2(a, first_unit, n_units, stream);
+    case 3: return launch_minors_b3(a, first_unit, n_units, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace pqp

@@ -33,67 +33,11 @@
 #include "dd.hpp"
 #include "perm_kernels.hpp"
 #include "pqp_types.hpp"
+#include "sweep_common.hpp"
 
 namespace pqp {
 
 namespace {
-
-__device__ __forceinline__ double fmaR(double a, double b, double c) { return __builtin_fma(a, b, c); }
-__device__ __forceinline__ float fmaR(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-
-// (ar + i ai) *= (xr + i xi): 2 multiplies + 2 FMAs.
-template <typename R>
-__device__ __forceinline__ void cmul(R &ar, R &ai, R xr, R xi) {
-  const R t1 = ai * xi;
-  const R t2 = ar * xi;
-  const R nr = fmaR(ar, xr, -t1);
-  const R ni = fmaR(ai, xr, t2);
-  ar = nr;
-  ai = ni;
-}
-
-// Read-only tables are addressed through the constant address space so
-// that wave-uniform accesses become scalar loads (s_load) regardless of the
-// stores the kernel makes to its output buffers.
-template <typename T>
-using cptr = const __attribute__((address_space(4))) T *;
-
-template <typename T>
-__device__ __forceinline__ cptr<T> as_const(const void *p, int64_t byte_off = 0) {
-  return reinterpret_cast<cptr<T>>(reinterpret_cast<uintptr_t>(p) + static_cast<uintptr_t>(byte_off));
-}
-
-template <typename T>
-__device__ __forceinline__ T uniform(T v) {
-  return v;
-}
-
-// Copy a (wave-uniform) value into a VGPR.  A VALU instruction reads at
-// most one scalar source; keeping coefficients in VGPRs lets the row data
-// stay scalar operands instead of being copied to VGPRs column by column.
-__device__ __forceinline__ double in_vgpr(double x) {
-  double r;
-  __asm__("v_mov_b64 %0, %1" : "=v"(r) : "s"(x));
-  return r;
-}
-__device__ __forceinline__ float in_vgpr(float x) {
-  float r;
-  __asm__("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
-  return r;
-}
-
-// 1/i for the exact binomial recurrence (i <= 64).
-__constant__ double kInvSmall[65] = {
-    0.0,        1.0 / 1,  1.0 / 2,  1.0 / 3,  1.0 / 4,  1.0 / 5,  1.0 / 6,  1.0 / 7,
-    1.0 / 8,    1.0 / 9,  1.0 / 10, 1.0 / 11, 1.0 / 12, 1.0 / 13, 1.0 / 14, 1.0 / 15,
-    1.0 / 16,   1.0 / 17, 1.0 / 18, 1.0 / 19, 1.0 / 20, 1.0 / 21, 1.0 / 22, 1.0 / 23,
-    1.0 / 24,   1.0 / 25, 1.0 / 26, 1.0 / 27, 1.0 / 28, 1.0 / 29, 1.0 / 30, 1.0 / 31,
-    1.0 / 32,   1.0 / 33, 1.0 / 34, 1.0 / 35, 1.0 / 36, 1.0 / 37, 1.0 / 38, 1.0 / 39,
-    1.0 / 40,   1.0 / 41, 1.0 / 42, 1.0 / 43, 1.0 / 44, 1.0 / 45, 1.0 / 46, 1.0 / 47,
-    1.0 / 48,   1.0 / 49, 1.0 / 50, 1.0 / 51, 1.0 / 52, 1.0 / 53, 1.0 / 54, 1.0 / 55,
-    1.0 / 56,   1.0 / 57, 1.0 / 58, 1.0 / 59, 1.0 / 60, 1.0 / 61, 1.0 / 62, 1.0 / 63,
-    1.0 / 64};
-__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // prod_j cs_j^{c_j}: all columns once (two interleaved chains for ILP),
 // then the extra powers; columns are sorted by multiplicity, so the
@@ -131,20 +75,6 @@ __device__ __forceinline__ void column_product(const R (&cr)[MCP], const R (&ci)
   pi = ai;
 }
 
-// Butterfly reduction of a complex dd value over the 64 lanes.
-__device__ __forceinline__ cdd wave_reduce(cdd v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    cdd o;
-    o.re.hi = __shfl_xor(v.re.hi, off);
-    o.re.lo = __shfl_xor(v.re.lo, off);
-    o.im.hi = __shfl_xor(v.im.hi, off);
-    o.im.lo = __shfl_xor(v.im.lo, off);
-    v = cdd_add(v, o);
-  }
-  return v;
-}
-
 // One wave unit: top states [64*local, 64*local+64) of item `item`.
 template <int MCP, typename R, bool COMP>
 __device__ void sweep_unit(const LaunchArgs &a, int item, int local) {
@@ -162,87 +92,39 @@ __device__ void sweep_unit(const LaunchArgs &a, int item, int local) {
   const bool active = u < U;
   if (!active) u = U - 1;
 
-  const cptr<R> B = as_const<R>(a.arena, it->b_off);
   const cptr<R> B2 = as_const<R>(a.arena, it->b2_off);
-  const cptr<double> alpha = as_const<double>(a.arena, it->alpha_off);
   const cptr<double> coef = as_const<double>(a.arena, it->coef_off);
   const cptr<uint32_t> steps = as_const<uint32_t>(a.arena, it->step_off);
   const cptr<int32_t> cpow = as_const<int32_t>(a.arena, it->cpow_off);
 
-  // Row sums at the lane's first state,
-  //   b_0 + sum_inner rho_k b_k + sum_top (rho_t - 2 g_t(u)) b_t,
-  // accumulated in one loop over the rows of B from the highest top digit
-  // down to the prepended row (one set of live row sums, one FMA block).
-  // Row data are scalar operands; the coefficient is always a VGPR so each
-  // update is a single v_fma with one scalar source.
-  //
-  // Top digits, mixed radix, top-down as the reference counter initialises
-  // (src/n_aryGrayCodeCounter.hpp:178-193): counter digit
-  // c_t = floor(u / P_t) - R_t floor(u / P_{t+1}); Gray digit g_t = c_t,
-  // reflected to R_t - 1 - c_t when the higher Gray digits have odd sum.
-  // floor(u / P) is evaluated as floor((u + 1/2) * (1/P)) in FP64, exact for
-  // u < 2^50 (the fraction stays >= 1/(2P) away from an integer).  For
-  // all-binary top digits this is the reflected binary code u ^ (u >> 1).
   R cr[MCP], ci[MCP];
-#pragma unroll
-  for (int j = 0; j < MCP; ++j) {
-    cr[j] = R(0);
-    ci[j] = R(0);
-  }
-  const cptr<double> top = as_const<double>(a.arena, it->top_off);
-  const uint64_t gbin = static_cast<uint64_t>(u) ^ (static_cast<uint64_t>(u) >> 1);
-  const double ud = static_cast<double>(u) + 0.5;
-  double q_hi = 0.0;
-  int parity = 0;
-  double w_top = 1.0;
-  for (int r = L + Dt; r >= 0; --r) {
-    R al;
-    if (r > L) {
-      const int t = r - L - 1;
-      if (flags & kTopBinary) {
-        al = ((gbin >> t) & 1) ? R(-1) : R(1);
-      } else {
-        const double inv_place = top[4 * t];
-        const double radix = top[4 * t + 1];
-        const int rho = static_cast<int>(top[4 * t + 2]);
-        const double q = __builtin_floor(ud * inv_place);
-        const int c = static_cast<int>(__builtin_fma(-radix, q_hi, q));
-        q_hi = q;
-        const int g = parity ? static_cast<int>(radix) - 1 - c : c;
-        parity ^= (g & 1);
-        // C(rho, g) = prod_{i=1..g} (rho - g + i) / i; every partial product
-        // is an integer, recovered exactly by rounding.
-        double cb = 1.0;
-        for (int i = 1; i <= rho; ++i) {
-          const double f = (i <= g) ? static_cast<double>(rho - g + i) : 1.0;
-          const double inv_i = (i <= g) ? kInvSmall[i] : 1.0;
-          cb = __builtin_rint(cb * f * inv_i);
-        }
-        w_top *= cb;
-        al = static_cast<R>(rho - 2 * g);
-      }
-    } else {
-      al = in_vgpr(r > 0 ? static_cast<R>(alpha[r - 1]) : R(1));
-    }
-    const cptr<R> row = B + static_cast<int64_t>(r) * 2 * MCP;
-#pragma unroll
-    for (int j = 0; j < MCP; ++j) {
-      cr[j] = fmaR(al, row[2 * j], cr[j]);
-      ci[j] = fmaR(al, row[2 * j + 1], ci[j]);
-    }
-  }
+  double w_top;
+  start_state<MCP, R>(a, it, u, L, Dt, flags, cr, ci, w_top);
   // (-1)^{|g_top|} = (-1)^u: consecutive top states differ by one unit step.
   double lane_w = (u & 1) ? -w_top : w_top;
   if (!active) lane_w = 0.0;
 
-  // Inner sweep.
+  // Inner sweep.  The row added after step v's product is fetched before
+  // that product (its first PF columns; the scalar registers hold at most
+  // 16 complex values), and the next step's row index one step ahead, so the
+  // scalar-load latency hides behind the product's VALU work.
+  constexpr int PF = MCP < 16 ? MCP : 16;
   double sr = 0.0, si = 0.0;
   dd accr{0.0, 0.0}, acci{0.0, 0.0};
   int fold = 0;
+  uint32_t st = steps[0];  // the table has S entries (the last one is padding)
   for (int64_t v = 0;; ++v) {
+    const cptr<R> row = B2 + static_cast<int64_t>(st) * 2 * MCP;
+    R ur[PF], ui[PF];
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+      ur[j] = row[2 * j];
+      ui[j] = row[2 * j + 1];
+    }
+    const uint32_t st_next = steps[v + 1 < S ? v + 1 : v];
+    const double cf = coef[v];
     R pr, pi;
     column_product<MCP, R>(cr, ci, pr, pi, ones, cmax, cpow);
-    const double cf = coef[v];
     sr = __builtin_fma(cf, static_cast<double>(pr), sr);
     si = __builtin_fma(cf, static_cast<double>(pi), si);
     if constexpr (COMP) {
@@ -255,12 +137,17 @@ __device__ void sweep_unit(const LaunchArgs &a, int item, int local) {
       }
     }
     if (v + 1 >= S) break;
-    const cptr<R> row = B2 + static_cast<int64_t>(steps[v]) * 2 * MCP;
 #pragma unroll
-    for (int j = 0; j < MCP; ++j) {
+    for (int j = 0; j < PF; ++j) {
+      cr[j] += ur[j];
+      ci[j] += ui[j];
+    }
+#pragma unroll
+    for (int j = PF; j < MCP; ++j) {
       cr[j] += row[2 * j];
       ci[j] += row[2 * j + 1];
     }
+    st = st_next;
   }
   cdd lane_sum;
   if constexpr (COMP) {
@@ -302,29 +189,25 @@ __device__ __forceinline__ void dispatch(const LaunchArgs &a, int mcp, int item,
 
 }  // namespace
 
-// Waves per SIMD the register allocator must leave room for: the row sums
-// take 4 VGPRs per column in FP64 (2 in FP32); the inner loop is pure VALU
-// with scalar operands, so 2-3 waves per SIMD hide the scalar-load latency.
-constexpr int waves_for(int hi, bool fp32) {
-  const int w = fp32 ? hi / 2 : hi;
-  return w <= 8 ? 6 : w <= 16 ? 4 : w <= 32 ? 3 : w <= 48 ? 2 : 1;
-}
-
 // Persistent, work-stealing launch: every wave loops over the queue.
 template <int LO, int HI, typename R, bool COMP>
 __global__ __launch_bounds__(kWave)
 __attribute__((amdgpu_waves_per_eu(waves_for(HI, sizeof(R) == 4), 8))) void perm_persistent_kernel(LaunchArgs a, int first_unit,
                                                                 int n_units, int head_slot) {
+  // The first unit of each wave is its block index; later units come from
+  // the queue, whose next index is requested before the current unit runs
+  // so the atomic's round trip overlaps the sweep.
+  int k = static_cast<int>(blockIdx.x);
   for (;;) {
-    int k = 0;
-    if (threadIdx.x == 0) k = atomicAdd(a.queue_head + head_slot, 1);
-    k = __builtin_amdgcn_readfirstlane(k);
     if (k >= n_units) break;
+    int next = 0;
+    if (threadIdx.x == 0) next = atomicAdd(a.queue_head + head_slot, 1);
     const cptr<WorkUnit> wu = as_const<WorkUnit>(a.units) + (first_unit + k);
     const int item = uniform(wu->item);
     const int local = uniform(wu->local);
     const int mcp = uniform(as_const<ItemDesc>(a.items)[item].mcp);
     dispatch<LO, HI, R, COMP>(a, mcp, item, local);
+    k = __builtin_amdgcn_readfirstlane(next) + static_cast<int>(gridDim.x);
   }
 }
 
@@ -395,11 +278,19 @@ __global__ __launch_bounds__(kWave) void perm_finalize_kernel(LaunchArgs a, int 
 }  // namespace pqp
 
 // ---------------------------------------------------------------------------
-// Instantiation for one column-width bucket (this file is compiled once per
-// bucket with -DPQP_BUCKET_ID=k; the bucket table matches plan.cpp).
+// Instantiation for one column-width bucket and one arithmetic variant.  This
+// file is compiled once per (bucket, variant) pair with -DPQP_BUCKET_ID=k and
+// -DPQP_VARIANT=v; the bucket table matches plan.cpp:
+//   buckets 0..3: widths 1..8, 9..16, 17..24, 25..32 (narrow ranges keep the
+//   register allocation, hence the waves per SIMD, close to each width's
+//   need); 4..7: 40, 48, 56, 64 (padded widths);
+//   variant 0: FP64 + double-double, 1: plain FP64, 2: FP32 products + dd.
 // ---------------------------------------------------------------------------
 #ifndef PQP_BUCKET_ID
 #error "compile with -DPQP_BUCKET_ID=<0..7>"
+#endif
+#ifndef PQP_VARIANT
+#error "compile with -DPQP_VARIANT=<0..2>"
 #endif
 
 #if defined(PQP_LO) && defined(PQP_HI)
@@ -430,82 +321,47 @@ __global__ __launch_bounds__(kWave) void perm_finalize_kernel(LaunchArgs a, int 
 #define PQP_HI 64
 #endif
 
-#define PQP_CAT2(a, b) a##b
-#define PQP_CAT(a, b) PQP_CAT2(a, b)
+#if PQP_VARIANT == 0
+#define PQP_R double
+#define PQP_COMP true
+#define PQP_SFX d
+#elif PQP_VARIANT == 1
+#define PQP_R double
+#define PQP_COMP false
+#define PQP_SFX p
+#else
+#define PQP_R float
+#define PQP_COMP true
+#define PQP_SFX f
+#endif
+
+#define PQP_PASTE_(base, k, v) base##k##_##v
+#define PQP_PASTE(base, k, v) PQP_PASTE_(base, k, v)
+#define PQP_NAME(base) PQP_PASTE(base, PQP_BUCKET_ID, PQP_SFX)
 
 namespace pqp {
 
-namespace {
-
-template <typename R, bool COMP>
-hipError_t launch_variant(const FwdLaunch &l) {
+hipError_t PQP_NAME(launch_fwd_b)(const FwdLaunch &l) {
   if (l.n_units <= 0) return hipSuccess;
   if (l.persistent) {
-    hipLaunchKernelGGL((perm_persistent_kernel<PQP_LO, PQP_HI, R, COMP>), dim3(l.grid),
-                       dim3(kWave), 0, l.stream, l.args, l.first_unit, l.n_units,
-                       l.head_slot);
+    hipLaunchKernelGGL((perm_persistent_kernel<PQP_LO, PQP_HI, PQP_R, PQP_COMP>), dim3(l.grid),
+                       dim3(kWave), 0, l.stream, l.args, l.first_unit, l.n_units, l.head_slot);
   } else {
-    hipLaunchKernelGGL((perm_static_kernel<PQP_LO, PQP_HI, R, COMP>), dim3(l.n_units),
+    hipLaunchKernelGGL((perm_static_kernel<PQP_LO, PQP_HI, PQP_R, PQP_COMP>), dim3(l.n_units),
                        dim3(kWave), 0, l.stream, l.args, l.first_unit);
   }
   return hipGetLastError();
 }
 
-template <typename R, bool COMP>
-hipError_t occupancy_variant(bool persistent, int *blocks) {
+hipError_t PQP_NAME(occupancy_fwd_b)(bool persistent, int *blocks) {
   if (persistent)
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        blocks, perm_persistent_kernel<PQP_LO, PQP_HI, R, COMP>, kWave, 0);
+        blocks, perm_persistent_kernel<PQP_LO, PQP_HI, PQP_R, PQP_COMP>, kWave, 0);
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      blocks, perm_static_kernel<PQP_LO, PQP_HI, R, COMP>, kWave, 0);
+      blocks, perm_static_kernel<PQP_LO, PQP_HI, PQP_R, PQP_COMP>, kWave, 0);
 }
 
-}  // namespace
-
-hipError_t PQP_CAT(launch_fwd_b, PQP_BUCKET_ID)(const FwdLaunch &l) {
-  if (l.fp32) {
-    if (!l.comp) return hipErrorInvalidValue;  // FP32 products always use dd accumulation
-    return launch_variant<float, true>(l);
-  }
-  return l.comp ? launch_variant<double, true>(l) : launch_variant<double, false>(l);
-}
-
-hipError_t PQP_CAT(occupancy_fwd_b, PQP_BUCKET_ID)(bool persistent, bool fp32, bool comp,
-                                                   int *blocks) {
-  if (fp32) return occupancy_variant<float, true>(persistent, blocks);
-  return comp ? occupancy_variant<double, true>(persistent, blocks)
-              : occupancy_variant<double, false>(persistent, blocks);
-}
-
-#if PQP_BUCKET_ID == 0
-hipError_t launch_fwd(int bucket, const FwdLaunch &l) {
-  switch (bucket) {
-    case 0: return launch_fwd_b0(l);
-    case 1: return launch_fwd_b1(l);
-    case 2: return launch_fwd_b2(l);
-    case 3: return launch_fwd_b3(l);
-    case 4: return launch_fwd_b4(l);
-    case 5: return launch_fwd_b5(l);
-    case 6: return launch_fwd_b6(l);
-    case 7: return launch_fwd_b7(l);
-    default: return hipErrorInvalidValue;
-  }
-}
-
-hipError_t occupancy_fwd(int bucket, bool persistent, bool fp32, bool comp, int *blocks) {
-  switch (bucket) {
-    case 0: return occupancy_fwd_b0(persistent, fp32, comp, blocks);
-    case 1: return occupancy_fwd_b1(persistent, fp32, comp, blocks);
-    case 2: return occupancy_fwd_b2(persistent, fp32, comp, blocks);
-    case 3: return occupancy_fwd_b3(persistent, fp32, comp, blocks);
-    case 4: return occupancy_fwd_b4(persistent, fp32, comp, blocks);
-    case 5: return occupancy_fwd_b5(persistent, fp32, comp, blocks);
-    case 6: return occupancy_fwd_b6(persistent, fp32, comp, blocks);
-    case 7: return occupancy_fwd_b7(persistent, fp32, comp, blocks);
-    default: return hipErrorInvalidValue;
-  }
-}
-
+#if PQP_BUCKET_ID == 0 && PQP_VARIANT == 0
 hipError_t launch_finalize(bool comp, const LaunchArgs &a, int item0, int n_items,
                            int n_heads, hipStream_t stream) {
   const int grid = n_items > 0 ? n_items : 1;

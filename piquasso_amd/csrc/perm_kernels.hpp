@@ -1,7 +1,7 @@
 // Host-side launch interface of the permanent kernels.  The kernel
-// templates are instantiated once per column-width bucket (perm_fwd.hip is
-// compiled once per bucket with -DPQP_BUCKET_ID=k); launch_fwd() routes a
-// launch to the right translation unit.
+// templates are instantiated once per column-width bucket and arithmetic
+// variant (perm_fwd.hip is compiled once per pair); launch_fwd() in
+// dispatch.cpp routes a launch to the right translation unit.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,10 +22,12 @@ struct FwdLaunch {
   hipStream_t stream = nullptr;
 };
 
-#define PQP_DECLARE_BUCKET(k)                                            \
-  hipError_t launch_fwd_b##k(const FwdLaunch &l);                        \
-  hipError_t occupancy_fwd_b##k(bool persistent, bool fp32, bool comp,   \
-                                int *blocks_per_cu);
+// Per (bucket, variant) entry points, defined in perm_fwd.hip builds;
+// variant suffix d = FP64 + dd, p = plain FP64, f = FP32 products + dd.
+#define PQP_DECLARE(k, v)                                         \
+  hipError_t launch_fwd_b##k##_##v(const FwdLaunch &l);           \
+  hipError_t occupancy_fwd_b##k##_##v(bool persistent, int *blocks_per_cu);
+#define PQP_DECLARE_BUCKET(k) PQP_DECLARE(k, d) PQP_DECLARE(k, p) PQP_DECLARE(k, f)
 PQP_DECLARE_BUCKET(0)
 PQP_DECLARE_BUCKET(1)
 PQP_DECLARE_BUCKET(2)
@@ -35,6 +37,18 @@ PQP_DECLARE_BUCKET(5)
 PQP_DECLARE_BUCKET(6)
 PQP_DECLARE_BUCKET(7)
 #undef PQP_DECLARE_BUCKET
+#undef PQP_DECLARE
+
+constexpr int kBuckets = 8;
+
+// Column-minor kernels (perm_minors.hip), buckets 0..3 (widths up to 32).
+hipError_t launch_minors_b0(const LaunchArgs &a, int first_unit, int n_units, hipStream_t s);
+hipError_t launch_minors_b1(const LaunchArgs &a, int first_unit, int n_units, hipStream_t s);
+hipError_t launch_minors_b2(const LaunchArgs &a, int first_unit, int n_units, hipStream_t s);
+hipError_t launch_minors_b3(const LaunchArgs &a, int first_unit, int n_units, hipStream_t s);
+hipError_t launch_minors(int bucket, const LaunchArgs &a, int first_unit, int n_units,
+                         hipStream_t stream);
+hipError_t launch_minors_finalize(const LaunchArgs &a, int n_items, hipStream_t stream);
 
 hipError_t launch_fwd(int bucket, const FwdLaunch &l);
 hipError_t occupancy_fwd(int bucket, bool persistent, bool fp32, bool comp, int *blocks_per_cu);

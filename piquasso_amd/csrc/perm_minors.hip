@@ -1,0 +1,294 @@
+// Column-minor ("Laplace") kernels for gfx950: for one row multiplicity
+// vector r and column multiplicities c with sum(c) = sum(r) + 1, every
+//
+//   M_l = perm(A; r, c - e_l) = 2^-(n-1) sum_g (-1)^{|g|} w(g)
+//                                 prod_k cs_k(g)^{c_k - [k = l]},   n = sum(r),
+//
+// in one Gray sweep (Lemma 1 of arXiv:2005.04214; the reference's
+// permanent_laplace_cpp, src/permanent_laplace.cpp:43-240, recomputes all
+// m_c products per term, O(m_c^2); here prefix and suffix products make it
+// O(m_c)).  The same launch also accumulates the uncut product
+// prod_k cs_k^{c_k}, which is what the reference returns for a column with
+// c_l = 0.
+//
+// The gradient of a permanent is built from these: d perm(A; r, c) / d A_ij
+// = r_i c_j perm(A; r - e_i, c - e_j), i.e. one column-minor item per row
+// with r_i > 0 (src/permanent.cpp:271-300 computes the same n^2 minors with
+// one full permanent each).
+//
+// Lane organisation, start state and Gray tables are those of the forward
+// kernels (perm_fwd.hip).  Per lane the row sums, the prefix products and
+// the m_c + 1 FP64 accumulators live in VGPRs; lane sums are scaled by the
+// lane's top weight and sign, reduced across the wave in double-double and
+// stored per work unit; a finalize kernel sums each item's units in a fixed
+// order (deterministic) and scales by 2^-(n-1).
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "dd.hpp"
+#include "perm_kernels.hpp"
+#include "pqp_types.hpp"
+#include "sweep_common.hpp"
+
+namespace pqp {
+
+namespace {
+
+template <int MCP, bool ONES>
+__device__ void minors_unit(const LaunchArgs &a, int item, int local) {
+  using R = double;
+  const cptr<ItemDesc> it = as_const<ItemDesc>(a.items) + item;
+  const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+  const int64_t U = it->U;
+  const int64_t S = it->S_in;
+  const int L = it->L;
+  const int Dt = it->Dt;
+  const int flags = it->flags;
+  constexpr bool ones = ONES;
+
+  int64_t u = static_cast<int64_t>(local) * kWave + lane;
+  const bool active = u < U;
+  if (!active) u = U - 1;
+
+  const cptr<R> B2 = as_const<R>(a.arena, it->b2_off);
+  const cptr<double> coef = as_const<double>(a.arena, it->coef_off);
+  const cptr<uint32_t> steps = as_const<uint32_t>(a.arena, it->step_off);
+  const cptr<int32_t> cmult = as_const<int32_t>(a.arena, it->cmul_off);
+
+  R cr[MCP], ci[MCP];
+  double w_top;
+  start_state<MCP, R>(a, it, u, L, Dt, flags, cr, ci, w_top);
+  double lane_w = (u & 1) ? -w_top : w_top;
+  if (!active) lane_w = 0.0;
+
+  double acr[MCP + 1], aci[MCP + 1];
+#pragma unroll
+  for (int k = 0; k <= MCP; ++k) {
+    acr[k] = 0.0;
+    aci[k] = 0.0;
+  }
+
+  constexpr int PF = MCP < 8 ? MCP : 8;  // row columns fetched ahead of the products
+  uint32_t st = steps[0];
+  for (int64_t v = 0;; ++v) {
+    const cptr<R> row = B2 + static_cast<int64_t>(st) * 2 * MCP;
+    R ur[PF], ui[PF];
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+      ur[j] = row[2 * j];
+      ui[j] = row[2 * j + 1];
+    }
+    const uint32_t st_next = steps[v + 1 < S ? v + 1 : v];
+    const double cf = coef[v];
+
+    // F_k = cs_k^{c_k - 1} (1 for unit multiplicity), E_k = F_k cs_k, both
+    // recomputed where needed (wave-uniform powers; no per-column arrays).
+    // Prefix P_k = prod_{k' < k} E_k', suffix Q_k = prod_{k' > k} E_k';
+    // minor product l: P_l F_l Q_l.
+    auto F = [&](int k, R &fr_, R &fi_) {
+      fr_ = cr[k];
+      fi_ = ci[k];
+      const int ck = cmult[k];
+      for (int q = 2; q < ck; ++q) cmul(fr_, fi_, cr[k], ci[k]);
+    };
+    auto E = [&](int k, R &er, R &ei) {
+      er = cr[k];
+      ei = ci[k];
+      if constexpr (!ONES) {
+        const int ck = cmult[k];
+        for (int q = 1; q < ck; ++q) cmul(er, ei, cr[k], ci[k]);
+      }
+    };
+    R pr[MCP], pi[MCP];
+    pr[0] = R(1);
+    pi[0] = R(0);
+    if constexpr (MCP > 1) {
+      E(0, pr[1], pi[1]);
+#pragma unroll
+      for (int k = 2; k < MCP; ++k) {
+        R er, ei;
+        E(k - 1, er, ei);
+        pr[k] = pr[k - 1];
+        pi[k] = pi[k - 1];
+        cmul(pr[k], pi[k], er, ei);
+      }
+    }
+    R qr = R(1), qi = R(0);
+#pragma unroll
+    for (int k = MCP - 1; k >= 0; --k) {
+      R mr, mi;
+      if (k == MCP - 1) {
+        mr = pr[k];
+        mi = pi[k];
+      } else if (k == 0) {
+        mr = qr;
+        mi = qi;
+      } else {
+        mr = pr[k];
+        mi = pi[k];
+        cmul(mr, mi, qr, qi);
+      }
+      if constexpr (!ONES) {
+        if (cmult[k] > 1) {
+          R fr_, fi_;
+          F(k, fr_, fi_);
+          cmul(mr, mi, fr_, fi_);
+        }
+      }
+      acr[k] = __builtin_fma(cf, mr, acr[k]);
+      aci[k] = __builtin_fma(cf, mi, aci[k]);
+      R er, ei;
+      E(k, er, ei);
+      if (k == MCP - 1) {
+        qr = er;
+        qi = ei;
+      } else {
+        cmul(qr, qi, er, ei);
+      }
+    }
+    // uncut product prod_k E_k = Q_{-1}
+    acr[MCP] = __builtin_fma(cf, qr, acr[MCP]);
+    aci[MCP] = __builtin_fma(cf, qi, aci[MCP]);
+
+    if (v + 1 >= S) break;
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+      cr[j] += ur[j];
+      ci[j] += ui[j];
+    }
+#pragma unroll
+    for (int j = PF; j < MCP; ++j) {
+      cr[j] += row[2 * j];
+      ci[j] += row[2 * j + 1];
+    }
+    st = st_next;
+  }
+
+  double *p = a.partials + 4 * (it->partial_base + static_cast<int64_t>(local) * (MCP + 1));
+#pragma unroll
+  for (int k = 0; k <= MCP; ++k) {
+    cdd s = cdd{{acr[k] * lane_w, 0.0}, {aci[k] * lane_w, 0.0}};
+    // the products above are exact for integer lane weights < 2^53 only up
+    // to rounding; carry the rounding error of the scaling into the low part
+    s.re.lo = __builtin_fma(acr[k], lane_w, -s.re.hi);
+    s.im.lo = __builtin_fma(aci[k], lane_w, -s.im.hi);
+    s = wave_reduce(s);
+    if (lane == 0) {
+      p[4 * k] = s.re.hi;
+      p[4 * k + 1] = s.re.lo;
+      p[4 * k + 2] = s.im.hi;
+      p[4 * k + 3] = s.im.lo;
+    }
+  }
+}
+
+template <int LO, int HI>
+__device__ __forceinline__ void minors_dispatch(const LaunchArgs &a, int mcp, bool ones,
+                                                int item, int local) {
+  if constexpr (LO <= HI) {
+    if (mcp == LO) {
+      if (ones)
+        minors_unit<LO, true>(a, item, local);
+      else
+        minors_unit<LO, false>(a, item, local);
+    } else {
+      minors_dispatch<LO + 1, HI>(a, mcp, ones, item, local);
+    }
+  }
+}
+
+}  // namespace
+
+// One block per work unit.
+template <int LO, int HI>
+__global__ __launch_bounds__(kWave)
+__attribute__((amdgpu_waves_per_eu(HI <= 8 ? 3 : 1, 8))) void perm_minors_kernel(
+    LaunchArgs a, int first_unit) {
+  const cptr<WorkUnit> wu = as_const<WorkUnit>(a.units) + (first_unit + static_cast<int>(blockIdx.x));
+  const int item = uniform(wu->item);
+  const int local = uniform(wu->local);
+  const cptr<ItemDesc> it = as_const<ItemDesc>(a.items) + item;
+  const int mcp = uniform(it->mcp);
+  const bool ones = (uniform(it->flags) & kColsAllOnes) != 0;
+  minors_dispatch<LO, HI>(a, mcp, ones, item, local);
+}
+
+// One wave per item; lane l sums column l's unit partials in unit order.
+__global__ __launch_bounds__(kWave) void perm_minors_finalize_kernel(LaunchArgs a) {
+  const int item = static_cast<int>(blockIdx.x);
+  const int lane = static_cast<int>(threadIdx.x);
+  if (item >= a.n_items) return;
+  const cptr<ItemDesc> it = as_const<ItemDesc>(a.items) + item;
+  const int mc = it->mc;
+  const int mcp = it->mcp;
+  const int units = it->units;
+  const int64_t base = it->partial_base;
+  for (int l = lane; l <= mc; l += kWave) {
+    // kernel slot mcp holds the uncut product; slots mc..mcp-1 are padding
+    const int slot = (l == mc) ? mcp : l;
+    cdd s = cdd_zero();
+    if (it->n == 0) {
+      s.re.hi = 1.0;  // perm of an empty selection
+    } else {
+      for (int k = 0; k < units; ++k) {
+        const double *p = a.partials + 4 * (base + static_cast<int64_t>(k) * (mcp + 1) + slot);
+        s = cdd_add(s, cdd{{p[0], p[1]}, {p[2], p[3]}});
+      }
+    }
+    const int e = it->n > 0 ? -(it->n - 1) : 0;
+    const double re_hi = ldexp(s.re.hi, e), re_lo = ldexp(s.re.lo, e);
+    const double im_hi = ldexp(s.im.hi, e), im_lo = ldexp(s.im.lo, e);
+    const int64_t o = static_cast<int64_t>(it->out_index) + l;
+    a.out[2 * o] = re_hi + re_lo;
+    a.out[2 * o + 1] = im_hi + im_lo;
+    if (a.out_dd) {
+      a.out_dd[4 * o] = re_hi;
+      a.out_dd[4 * o + 1] = re_lo;
+      a.out_dd[4 * o + 2] = im_hi;
+      a.out_dd[4 * o + 3] = im_lo;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Instantiation per column-width bucket (-DPQP_BUCKET_ID=0..3, widths up to 32).
+// ---------------------------------------------------------------------------
+#ifndef PQP_BUCKET_ID
+#error "compile with -DPQP_BUCKET_ID=<0..3>"
+#endif
+#if PQP_BUCKET_ID == 0
+#define PQP_LO 1
+#define PQP_HI 8
+#elif PQP_BUCKET_ID == 1
+#define PQP_LO 9
+#define PQP_HI 16
+#elif PQP_BUCKET_ID == 2
+#define PQP_LO 17
+#define PQP_HI 24
+#elif PQP_BUCKET_ID == 3
+#define PQP_LO 25
+#define PQP_HI 32
+#endif
+
+#define PQP_PASTE_(base, k) base##k
+#define PQP_PASTE(base, k) PQP_PASTE_(base, k)
+
+hipError_t PQP_PASTE(launch_minors_b, PQP_BUCKET_ID)(const LaunchArgs &a, int first_unit,
+                                                      int n_units, hipStream_t stream) {
+  if (n_units <= 0) return hipSuccess;
+  hipLaunchKernelGGL((perm_minors_kernel<PQP_LO, PQP_HI>), dim3(n_units), dim3(kWave), 0, stream,
+                     a, first_unit);
+  return hipGetLastError();
+}
+
+#if PQP_BUCKET_ID == 0
+hipError_t launch_minors_finalize(const LaunchArgs &a, int n_items, hipStream_t stream) {
+  if (n_items <= 0) return hipSuccess;
+  hipLaunchKernelGGL(perm_minors_finalize_kernel, dim3(n_items), dim3(kWave), 0, stream, a);
+  return hipGetLastError();
+}
+#endif
+
+}  // namespace pqp

@@ -12,6 +12,7 @@ namespace {
 
 constexpr int64_t kMaxTerms = int64_t(1) << 62;
 constexpr int64_t kMaxInner = int64_t(1) << 16;   // step/coef table length cap
+constexpr int kExpandAutoMax = 0;                 // auto mode: never (measured slower)
 
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
@@ -209,7 +210,32 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     std::stable_sort(colsel.begin(), colsel.end(), [&](int64_t x, int64_t y) {
       return in.cols[x] > in.cols[y];
     });
-    const int mc = static_cast<int>(colsel.size());
+    const int mc_alg = static_cast<int>(colsel.size());  // algorithmic column count
+    int cmax_in = 0;
+    for (int64_t j : colsel) cmax_in = std::max(cmax_in, static_cast<int>(in.cols[j]));
+    // Kernel columns: a column of multiplicity c either stays one column
+    // whose row sum is raised to the power c (extra multiplies behind
+    // wave-uniform branches), or is expanded into c identical columns of
+    // multiplicity 1 (branch-free product, 2 (c - 1) more additions per term).
+    bool expand = false;
+    if (cmax_in > 1) {
+      if (opt.expand_cols == 1) expand = n <= kMaxCols;
+      if (opt.expand_cols == 2) expand = n <= kExpandAutoMax;
+    }
+    std::vector<int64_t> colsrc;  // source column of each kernel column
+    std::vector<int> colmult;     // multiplicity of each kernel column
+    for (int64_t j : colsel) {
+      if (expand) {
+        for (int q = 0; q < in.cols[j]; ++q) {
+          colsrc.push_back(j);
+          colmult.push_back(1);
+        }
+      } else {
+        colsrc.push_back(j);
+        colmult.push_back(in.cols[j]);
+      }
+    }
+    const int mc = static_cast<int>(colsrc.size());
     int mcp = 0;
     const int bucket = bucket_for_width(mc, opt.fp32_products, &mcp);
     if (bucket < 0) {
@@ -253,7 +279,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
       return digits[x].rho < digits[y].rho;
     });
     int cmax = 0;
-    for (int64_t j : colsel) cmax = std::max(cmax, static_cast<int>(in.cols[j]));
+    for (int c : colmult) cmax = std::max(cmax, c);
     const double step_cost = 2.0 * mc + 4.0 * (n - 1) + 2.0 + 6.0 +
                              (cmax > 1 ? 2.0 * (n - mc) : 0.0);
     int best_L = 0;
@@ -339,7 +365,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
       for (int j = 0; j < mcp; ++j) {
         double re = 0.0, im = 0.0;
         if (j < mc)
-          entry(src_rows[r], colsel[j], re, im);
+          entry(src_rows[r], colsrc[j], re, im);
         else if (r == 0)
           re = 1.0;  // padding column: row sum stays exactly 1 + 0i
         put(d.b_off, r, j, re, im);
@@ -349,7 +375,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     for (int t = 0; t < L; ++t)
       for (int j = 0; j < mc; ++j) {
         double re, im;
-        entry(digits[inner[t]].src_row, colsel[j], re, im);
+        entry(digits[inner[t]].src_row, colsrc[j], re, im);
         put(d.b2_off, 2 * t, j, 2.0 * re, 2.0 * im);
         put(d.b2_off, 2 * t + 1, j, -2.0 * re, -2.0 * im);
       }
@@ -367,7 +393,8 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
       for (int r : profile) radix.push_back(r + 1);
       std::vector<int> sd, sv;
       gray_steps(radix, sd, sv);
-      tb.step_off = arena_alloc(plan.arena, int64_t(std::max<size_t>(sd.size(), 1)) * 4);
+      // S entries: the kernel reads one entry ahead, the last one is padding
+      tb.step_off = arena_alloc(plan.arena, (static_cast<int64_t>(sd.size()) + 1) * 4);
       tb.coef_off = arena_alloc(plan.arena, S_in * 8);
       std::vector<int> g(L, 0);
       int64_t w = 1;
@@ -408,7 +435,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     d.cpow_off = arena_alloc(plan.arena, int64_t(std::max(cmax - 1, 1)) * 4);
     for (int r = 2; r <= cmax; ++r) {
       int cnt = 0;
-      for (int64_t j : colsel) cnt += in.cols[j] >= r;
+      for (int c : colmult) cnt += c >= r;
       at<int32_t>(plan.arena, d.cpow_off)[r - 2] = cnt;
     }
 
@@ -428,7 +455,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
 
     ItemInfo &inf = plan.info[k];
     inf.terms = T;
-    inf.flops = static_cast<double>(T) * (2.0 * mc + 6.0 * (n - 1) + 4.0);
+    inf.flops = static_cast<double>(T) * (2.0 * mc_alg + 6.0 * (n - 1) + 4.0);
     inf.unit_cost = ((1.0 + D) * 2.0 * mcp + Dt * 40.0 + 400.0) +
                     static_cast<double>(S_in) * step_cost;
     plan.total_flops += inf.flops;
@@ -456,7 +483,8 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     if (item_units[k] == 0) continue;
     const int b = item_bucket[k];
     if (plan.ranges.empty() || plan.ranges.back().bucket != b)
-      plan.ranges.push_back(Plan::Range{b, static_cast<int32_t>(plan.units.size()), 0});
+      plan.ranges.push_back(Plan::Range{b, static_cast<int32_t>(plan.units.size()), 0, 0.0});
+    plan.ranges.back().cost += plan.info[k].unit_cost * static_cast<double>(item_units[k]);
     for (int64_t l = 0; l < item_units[k]; ++l) {
       plan.units.push_back(WorkUnit{static_cast<int32_t>(k), static_cast<int32_t>(l)});
       plan.bucket_of_unit.push_back(b);

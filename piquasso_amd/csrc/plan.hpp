@@ -40,6 +40,9 @@ struct PlanOptions {
   bool fp32_products = false;   // FP32 row sums and products (tables in float)
   int64_t target_inner = 0;     // requested inner states per lane (0 = cost model)
   int64_t min_units = 0;        // split every permanent into at least this many waves
+  int expand_cols = 2;          // columns with multiplicity c > 1: 0 keep (power c),
+                                // 1 expand into c columns, 2 planner's choice (keep:
+                                // expansion widens the kernel and lowers occupancy)
   int64_t total_units = 0;      // spread about this many waves over the batch, in
                                 // proportion to each permanent's cost (0 = off)
 };
@@ -61,7 +64,7 @@ struct Plan {
   std::vector<WorkUnit> units;      // LPT order within each width bucket
   std::vector<int> bucket_of_unit;  // kernel bucket id per unit
   // contiguous unit ranges per kernel bucket: {bucket id, first unit, count}
-  struct Range { int bucket; int32_t first; int32_t count; };
+  struct Range { int bucket; int32_t first; int32_t count; double cost; };
   std::vector<Range> ranges;
   int64_t n_partials = 0;
   bool fp32 = false;

@@ -30,6 +30,7 @@ constexpr int kDdFold = 64;        // FP64 run length folded into dd accumulator
 enum ItemFlags : int {
   kColsAllOnes = 1,   // every column multiplicity is 1: plain product
   kTopBinary = 2,     // every top digit has radix 2 (rho = 1): bit-level Gray
+  kMinors = 4,        // column-minor item: perm(A; r, c - e_l) for every column l
 };
 
 // One permanent; read by the kernels with scalar loads (the item index is
@@ -44,6 +45,7 @@ struct alignas(16) ItemDesc {
   int64_t top_off;      // double x 4 per top digit: {1 / P_t, radix, rho, 0} with place
                         //    value P_t = prod_{t' < t} radix_t'
   int64_t cpow_off;     // int32 per r = 2..cmax: number of columns with c_j >= r
+  int64_t cmul_off;     // int32 per kernel column: its multiplicity c_j (minors kernel)
   int64_t S_in;         // inner states per lane
   int64_t U;            // top states (lanes of work)
   int64_t partial_base; // first partial slot of this permanent
@@ -55,7 +57,7 @@ struct alignas(16) ItemDesc {
   int32_t Dt;           // top digits
   int32_t flags;        // ItemFlags
   int32_t cmax;         // largest column multiplicity
-  int32_t out_index;    // result slot
+  int32_t out_index;    // result slot (minors: first of mc + 1 consecutive slots)
   int32_t pad0;
 };
 static_assert(sizeof(ItemDesc) == 128, "ItemDesc layout");

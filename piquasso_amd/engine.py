@@ -79,6 +79,13 @@ class PermanentBatch:
     target_inner:
         inner Gray states per lane (0 lets the planner's cost model choose;
         the autotuner in :mod:`piquasso_amd.tune` picks it per workload).
+    expand_cols:
+        None (planner's choice, currently keep), True (columns of multiplicity
+        c > 1 become c identical columns) or False (keep one column whose row
+        sum is raised to the power c).
+    total_units:
+        wave work units spread over the batch in proportion to each
+        permanent's cost (0: 48 per compute unit).
     """
 
     def __init__(
@@ -89,6 +96,8 @@ class PermanentBatch:
         precision: str = "fp64",
         persistent: bool = True,
         target_inner: int = 0,
+        expand_cols=None,
+        total_units: int = 0,
     ):
         if not (len(matrices) == len(rows) == len(cols)):
             raise ValueError("matrices, rows and cols must have the same length")
@@ -115,13 +124,17 @@ class PermanentBatch:
             flags |= _capi.PQP_FLAG_FP32
         if not persistent:
             flags |= _capi.PQP_FLAG_STATIC
+        if expand_cols is True:
+            flags |= _capi.PQP_FLAG_EXPAND_COLS
+        elif expand_cols is False:
+            flags |= _capi.PQP_FLAG_KEEP_COLS
         handle = ctypes.c_void_p()
         L = _capi.lib()
         st = L.pqp_batch_create(
             n, mat_arr, 0, m_r.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
             m_c.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
             m_c.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
-            row_arr, col_arr, flags, int(target_inner), ctypes.byref(handle),
+            row_arr, col_arr, flags, int(target_inner), int(total_units), ctypes.byref(handle),
         )
         _capi.check(st)
         self._h = handle

@@ -1,0 +1,124 @@
+"""Autotuner for the permanent engine's launch parameters.
+
+The planner splits each permanent into wave work units (64 top Gray states
+per wave, an inner Gray sweep per lane).  How finely to split, whether to
+expand multiplicity columns, and whether to use the persistent work-stealing
+launch or one block per unit are workload dependent.  This module measures
+candidate settings on the device (HIP-event timing of the sweep kernels) and
+writes the fastest per workload class to a JSON table that
+:func:`load_tuning` reads back.
+
+CLI:  python -m piquasso_amd.tune [--out piquasso_amd/tuning/gfx950.json] [--reps 5]
+"""
+
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import os
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_TABLE = os.path.join(HERE, "tuning", "gfx950.json")
+
+
+def measure(mats, rows, cols, reps: int = 5, mode: str = "kernel", **cfg) -> dict:
+    """Median time of the batch under one configuration (PermanentBatch kwargs)."""
+    from .engine import PermanentBatch
+
+    t0 = time.perf_counter()
+    with PermanentBatch(mats, rows, cols, **cfg) as b:
+        plan_s = time.perf_counter() - t0
+        b.run("eager")
+        b.sync()
+        ms = b.time(mode, reps=reps)
+        st = b.stats()
+    return {"config": cfg, "ms": float(np.median(ms)), "ms_all": ms.tolist(),
+            "plan_s": plan_s, "units": st["units"], "flops": st["flops"],
+            "tflops": st["flops"] / (float(np.median(ms)) * 1e-3) / 1e12}
+
+
+def sweep(mats, rows, cols, grid: dict, reps: int = 5, mode: str = "kernel", log=print):
+    """Measure every combination of `grid` (dict of PermanentBatch kwarg ->
+    candidate values); returns the records sorted fastest first."""
+    keys = sorted(grid)
+    recs = []
+    for values in itertools.product(*[grid[k] for k in keys]):
+        cfg = dict(zip(keys, values))
+        rec = measure(mats, rows, cols, reps=reps, mode=mode, **cfg)
+        log(json.dumps({"config": cfg, "ms": round(rec["ms"], 4), "tflops": round(rec["tflops"], 2),
+                        "units": rec["units"]}))
+        recs.append(rec)
+    recs.sort(key=lambda r: r["ms"])
+    return recs
+
+
+def _workloads():
+    from . import workloads as W
+
+    mats, rows, cols = W.batched_uneven(count=4096, seed=4)
+    a32, r32, c32 = W.large_single(32, seed=3)
+    a24, r24, c24 = W.large_single(24, seed=3)
+    g_m, g_r, g_c = W.gradient_batch(count=256, seed=5)
+    return {
+        "batched_uneven": (mats, rows, cols),
+        "single_n32": ([a32], [r32], [c32]),
+        "single_n24": ([a24], [r24], [c24]),
+        "uniform_n12_20": (g_m, g_r, g_c),
+    }
+
+
+GRIDS = {
+    "batched_uneven": {"total_units": [4096, 8192, 12288, 24576, 49152],
+                       "expand_cols": [None, False], "persistent": [True, False]},
+    "single_n32": {"total_units": [4096, 8192, 16384, 32768], "persistent": [False, True]},
+    "single_n24": {"total_units": [2048, 4096, 8192, 16384], "persistent": [False, True]},
+    "uniform_n12_20": {"total_units": [4096, 12288, 24576], "persistent": [True, False]},
+}
+
+
+def tune(out: str = DEFAULT_TABLE, reps: int = 5, only=None, log=print) -> dict:
+    from . import _capi
+
+    n, cus, arch = _capi.device_info()
+    table = {"device": arch, "cus": cus, "workloads": {}}
+    for name, (mats, rows, cols) in _workloads().items():
+        if only and name not in only:
+            continue
+        log(f"== {name}")
+        recs = sweep(mats, rows, cols, GRIDS[name], reps=reps, log=log)
+        best = recs[0]
+        table["workloads"][name] = {
+            "best": best["config"], "ms": best["ms"], "tflops": best["tflops"],
+            "units_per_cu": best["config"].get("total_units", 0) / cus,
+            "all": [{"config": r["config"], "ms": r["ms"]} for r in recs],
+        }
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    with open(out, "w") as f:
+        json.dump(table, f, indent=1)
+    log(f"wrote {out}")
+    return table
+
+
+def load_tuning(path: str = DEFAULT_TABLE) -> dict:
+    """The tuned table, or {} when none has been written for this device."""
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--out", default=DEFAULT_TABLE)
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--only", nargs="*", default=None)
+    args = ap.parse_args()
+    tune(args.out, args.reps, args.only, log=lambda s: print(s, flush=True))
+
+
+if __name__ == "__main__":
+    main()

@@ -25,7 +25,7 @@ ITEM = np.dtype([
 assert ITEM.itemsize == 128
 
 
-def plan_image(a, rows, cols, target_inner=0, min_units=0):
+def plan_image(a, rows, cols, target_inner=0, min_units=0, flags=0):
     a = np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
     r = np.ascontiguousarray(np.asarray(rows, dtype=np.int32))
     c = np.ascontiguousarray(np.asarray(cols, dtype=np.int32))
@@ -34,7 +34,7 @@ def plan_image(a, rows, cols, target_inner=0, min_units=0):
     dp = ctypes.POINTER(ctypes.c_double)
     ip = ctypes.POINTER(ctypes.c_int32)
     args = (a.view(np.float64).ctypes.data_as(dp), a.shape[0], a.shape[1], a.shape[1],
-            r.ctypes.data_as(ip), c.ctypes.data_as(ip), 0, target_inner, min_units)
+            r.ctypes.data_as(ip), c.ctypes.data_as(ip), flags, target_inner, min_units)
     _capi.check(L.pqp_plan_image(*args, None, 0, ctypes.byref(need)))
     buf = np.zeros(need.value, dtype=np.uint8)
     _capi.check(L.pqp_plan_image(*args, buf.ctypes.data, need.value, ctypes.byref(need)))
@@ -101,8 +101,9 @@ def test_plan_emulation_matches_oracle(oracle):
         c = np.bincount(rng.integers(0, m_c, n), minlength=m_c)
         ref = oracle.perm_dd(a, r, c)
         for target in (0, 1, 2, 3, 8, 1 << 20):
-            got = emulate(plan_image(a, r, c, target_inner=target))
-            assert abs(got - ref) <= 1e-11 * max(abs(ref), 1e-300), (trial, target, got, ref)
+            for flags in (0, _capi.PQP_FLAG_EXPAND_COLS, _capi.PQP_FLAG_KEEP_COLS):
+                got = emulate(plan_image(a, r, c, target_inner=target, flags=flags))
+                assert abs(got - ref) <= 1e-11 * max(abs(ref), 1e-300), (trial, target, flags)
 
 
 def test_plan_splits_large_items(oracle):
@@ -117,6 +118,15 @@ def test_plan_splits_large_items(oracle):
     assert d["flags"] & 2  # all top digits binary
     ref = oracle.perm_dd(a, ones, ones)
     assert abs(emulate(buf) - ref) <= 1e-11 * abs(ref)
+
+
+def test_column_expansion_layout():
+    a = np.arange(6, dtype=float).reshape(2, 3) + 0j
+    r, c = [2, 3], [3, 0, 2]
+    kept = np.frombuffer(plan_image(a, r, c, flags=_capi.PQP_FLAG_KEEP_COLS)[:128].tobytes(), ITEM)[0]
+    exp = np.frombuffer(plan_image(a, r, c, flags=_capi.PQP_FLAG_EXPAND_COLS)[:128].tobytes(), ITEM)[0]
+    assert (kept["mc"], kept["cmax"], kept["flags"] & 1) == (2, 3, 0)
+    assert (exp["mc"], exp["cmax"], exp["flags"] & 1) == (5, 1, 1)
 
 
 def test_padding_columns_are_neutral(oracle):
