@@ -215,6 +215,7 @@ __attribute__((amdgpu_waves_per_eu(HI <= 8 ? 3 : 1, 8))) void perm_minors_kernel
   minors_dispatch<LO, HI>(a, mcp, ones, item, local);
 }
 
+#if PQP_BUCKET_ID == 0
 // One wave per item; lane l sums column l's unit partials in unit order.
 __global__ __launch_bounds__(kWave) void perm_minors_finalize_kernel(LaunchArgs a) {
   const int item = static_cast<int>(blockIdx.x);
@@ -251,6 +252,8 @@ __global__ __launch_bounds__(kWave) void perm_minors_finalize_kernel(LaunchArgs 
     }
   }
 }
+
+#endif
 
 // ---------------------------------------------------------------------------
 // Instantiation per column-width bucket (-DPQP_BUCKET_ID=0..3, widths up to 32).

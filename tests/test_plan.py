@@ -17,10 +17,10 @@ from piquasso_amd import _capi
 
 ITEM = np.dtype([
     ("b_off", "<i8"), ("b2_off", "<i8"), ("step_off", "<i8"), ("coef_off", "<i8"),
-    ("alpha_off", "<i8"), ("top_off", "<i8"), ("cpow_off", "<i8"), ("S_in", "<i8"),
-    ("U", "<i8"), ("partial_base", "<i8"), ("units", "<i4"), ("mcp", "<i4"), ("mc", "<i4"),
-    ("n", "<i4"), ("L", "<i4"), ("Dt", "<i4"), ("flags", "<i4"), ("cmax", "<i4"),
-    ("out_index", "<i4"), ("pad0", "<i4"), ("tail", "V8"),
+    ("alpha_off", "<i8"), ("top_off", "<i8"), ("cpow_off", "<i8"), ("cmul_off", "<i8"),
+    ("S_in", "<i8"), ("U", "<i8"), ("partial_base", "<i8"), ("units", "<i4"), ("mcp", "<i4"),
+    ("mc", "<i4"), ("n", "<i4"), ("L", "<i4"), ("Dt", "<i4"), ("flags", "<i4"), ("cmax", "<i4"),
+    ("out_index", "<i4"), ("pad0", "<i4"),
 ])
 assert ITEM.itemsize == 128
 
