@@ -51,6 +51,8 @@ extern "C" {
  * identical columns instead (branch-free product, wider kernel). */
 #define PQP_FLAG_EXPAND_COLS 0x8u
 #define PQP_FLAG_KEEP_COLS 0x10u
+/* Batches: also evaluate d perm / d A for every permanent (pqp_batch_gradients). */
+#define PQP_FLAG_GRADIENT 0x20u
 
 /* Library version string. */
 const char *pqp_version(void);
@@ -134,14 +136,46 @@ int pqp_batch_time(pqp_batch *b, int mode, int reps, float *ms);
 int pqp_batch_stats(const pqp_batch *b, double *flops, int64_t *terms, int64_t *units,
                     int64_t *arena_bytes);
 
+/* Gradient batches (PQP_FLAG_GRADIENT): grads[k] receives the m_r x m_c
+ * complex matrix d perm_k / d A_k (row-major, interleaved) of the last run. */
+int pqp_batch_gradients(pqp_batch *b, double *const *grads);
+
 void pqp_batch_destroy(pqp_batch *b);
+
+/*
+ * Column minors, complex128: out[l] = perm(A; rows, cols - e_l) for every
+ * column l with cols[l] > 0, where sum(cols) = sum(rows) + 1, all from one
+ * Gray sweep per work unit (prefix / suffix products).  Replaces
+ * permanent_laplace_cpp<double> (src/permanent_laplace.cpp:43-240) as reached
+ * from the pybind entry permanent_laplace(matrix, rows, cols)
+ * (piquasso/_math/permanent.cpp:43-58,82-84).  For a column with cols[l] = 0
+ * the value is the uncut sum (the reference computes the same there); with
+ * all row multiplicities 0 every entry is 1.  out: 2 * m_c doubles.
+ */
+int pqp_laplace_c128(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
+                     const int32_t *rows, const int32_t *cols, double *out);
+
+/*
+ * Gradients of `count` permanents: grads[k] (m_r[k] x m_c[k] complex,
+ * row-major, interleaved) = d perm(A_k; r_k, c_k) / d (A_k)_ij
+ * = r_i c_j perm(A_k; r_k - e_i, c_k - e_j) (0 where r_i or c_j is 0), the
+ * result of grad_perm (src/permanent.cpp:271-300) and of the JAX extension's
+ * backward (src/jax_perm/jax_perm_core.cpp:141-208), here without the
+ * square-matrix restriction.  All minors of all permanents run in one
+ * launch of column-minor sweeps (one per row of non-zero multiplicity).
+ * perms (2 * count doubles, may be null) receives the permanents.
+ */
+int pqp_grad_batch_c128(int64_t count, const double *const *matrices, const int64_t *m_r,
+                        const int64_t *m_c, const int64_t *ld, const int32_t *const *rows,
+                        const int32_t *const *cols, double *perms, double *const *grads);
 
 /*
  * Planning only (no device involved): the device arena the kernels would
  * read for one permanent, for inspection by tests of the planner.  Layout:
  * the ItemDesc record at offset 0, then the work units and tables at the
  * offsets it names (piquasso_amd/csrc/pqp_types.hpp).  With buf == null only
- * *needed_bytes is written.  No counterpart in the reference.
+ * *needed_bytes is written.  flags & PQP_FLAG_GRADIENT plans the column-minor
+ * sweep of pqp_laplace_c128 instead.  No counterpart in the reference.
  */
 int pqp_plan_image(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
                    const int32_t *rows, const int32_t *cols, uint32_t flags,

@@ -185,8 +185,8 @@ static cdd_t dd_glynn_sum(const double *b, int nr, const int *rho, int mc,
 /* Compact rows/columns with zero multiplicity, apply the reference prepend
  * rule on the compacted rows (same row chosen: the first minimal non-zero
  * one), then evaluate.  out = {re_hi, re_lo, im_hi, im_lo}. */
-static int dd_perm(const double *a, int m_r, int m_c, const int *rows,
-                   const int *cols, int n_jobs, double *out) {
+static int dd_perm_any(const double *a, int m_r, int m_c, const int *rows,
+                       const int *cols, int n_jobs, int check_sums, double *out) {
   long sr = 0, sc = 0;
   for (int i = 0; i < m_r; ++i) {
     if (rows[i] < 0) return -2;
@@ -196,7 +196,7 @@ static int dd_perm(const double *a, int m_r, int m_c, const int *rows,
     if (cols[j] < 0) return -2;
     sc += cols[j];
   }
-  if (sr != sc) return -1;
+  if (check_sums && sr != sc) return -1;
   if (sr == 0) {
     out[0] = 1.0;
     out[1] = out[2] = out[3] = 0.0;
@@ -249,10 +249,47 @@ static int dd_perm(const double *a, int m_r, int m_c, const int *rows,
   return st;
 }
 
+static int dd_perm(const double *a, int m_r, int m_c, const int *rows,
+                   const int *cols, int n_jobs, double *out) {
+  return dd_perm_any(a, m_r, m_c, rows, cols, n_jobs, 1, out);
+}
+
 int pqo_perm_dd_c128(const double *a, int m_r, int m_c, const int *rows,
                      const int *cols, int n_jobs, double *out) {
   if (m_r < 0 || m_c < 0 || !out) return -2;
   return dd_perm(a, m_r, m_c, rows, cols, n_jobs, out);
+}
+
+/* permanent_laplace_cpp<double> (src/permanent_laplace.cpp:43-240) as
+ * separate double-double sums: out[l] = perm(A; r, c - e_l) for c_l > 0 and,
+ * for c_l = 0, the Glynn sum with the uncut column multiplicities, scaled by
+ * 2^-(sum r - 1) (the reference's loop runs the product over all columns
+ * there, :206-212, and divides by that power of two, :228).  Requires
+ * sum c = sum r + 1 and sum r > 0; out is m_c interleaved complex values. */
+int pqo_laplace_dd_c128(const double *a, int m_r, int m_c, const int *rows,
+                        const int *cols, int n_jobs, double *out) {
+  if (m_r <= 0 || m_c <= 0 || !out) return -2;
+  long sr = 0, sc = 0;
+  for (int i = 0; i < m_r; ++i) sr += rows[i];
+  for (int j = 0; j < m_c; ++j) sc += cols[j];
+  if (sr <= 0 || sc != sr + 1) return -1;
+  int *c2 = (int *)malloc((size_t)m_c * sizeof(int));
+  memcpy(c2, cols, (size_t)m_c * sizeof(int));
+  int st = 0;
+  for (int l = 0; l < m_c && st == 0; ++l) {
+    double v[4];
+    if (cols[l] > 0) {
+      c2[l] -= 1;
+      st = dd_perm(a, m_r, m_c, rows, c2, n_jobs, v);
+      c2[l] += 1;
+    } else {
+      st = dd_perm_any(a, m_r, m_c, rows, cols, n_jobs, 0, v);
+    }
+    out[2 * l] = v[0] + v[1];
+    out[2 * l + 1] = v[2] + v[3];
+  }
+  free(c2);
+  return st;
 }
 
 int pqo_grad_dd_c128(const double *a, int m_r, int m_c, const int *rows,

@@ -57,6 +57,12 @@ int pqo_perm_dd_c128(const double *a, int m_r, int m_c, const int *rows,
 int pqo_grad_dd_c128(const double *a, int m_r, int m_c, const int *rows,
                      const int *cols, int n_jobs, double *g);
 
+/* Column minors in double-double (permanent_laplace_cpp<double>,
+ * src/permanent_laplace.cpp:43-240): out[l] = perm(A; r, c - e_l) for
+ * c_l > 0, the uncut Glynn sum for c_l = 0.  Needs sum c = sum r + 1 > 1. */
+int pqo_laplace_dd_c128(const double *a, int m_r, int m_c, const int *rows,
+                        const int *cols, int n_jobs, double *out);
+
 /* Mixed-radix reflected Gray code, restating n_aryGrayCodeCounter
  * (src/n_aryGrayCodeCounter.hpp:170-254).  Writes the digit vectors of the
  * Gray states with indices [offset, offset + count) into `codes`

@@ -17,6 +17,8 @@ Contents
     arbiter;
   - :func:`grad_dd` — ``grad_perm`` (src/permanent.cpp:271-300) as explicit
     minors in double-double;
+  - :func:`laplace_dd` — ``permanent_laplace_cpp`` (src/permanent_laplace.cpp:43-240)
+    as separate double-double sums per column;
   - :func:`gray_states` — the reflected n-ary Gray counter
     (src/n_aryGrayCodeCounter.hpp:170-254).
 
@@ -76,6 +78,7 @@ def lib():
             ("pqo_perm_port_c64", [fp, ci, ci, ip, ip, ci, fp]),
             ("pqo_perm_dd_c128", [dp, ci, ci, ip, ip, ci, dp]),
             ("pqo_grad_dd_c128", [dp, ci, ci, ip, ip, ci, dp]),
+            ("pqo_laplace_dd_c128", [dp, ci, ci, ip, ip, ci, dp]),
             ("pqo_gray_states", [ip, ci, ctypes.c_int64, ctypes.c_int64, ip]),
         ):
             fn = getattr(L, name)
@@ -169,6 +172,38 @@ def grad_dd(matrix, rows, cols, n_jobs: int = 0) -> np.ndarray:
         )
     )
     return g
+
+
+def laplace_dd(matrix, rows, cols, n_jobs: int = 0) -> np.ndarray:
+    """Column minors of ``permanent_laplace_cpp`` (src/permanent_laplace.cpp:43-240)
+    in double-double: perm(A; r, c - e_l) for c_l > 0, the uncut Glynn sum for
+    c_l = 0.  Degenerate inputs (empty matrix, a zero multiplicity sum) give
+    the reference's ``[1]`` (src/permanent_laplace.cpp:52-57)."""
+    a, r, c = _prep(matrix, rows, cols)
+    if a.shape[0] == 0 or a.shape[1] == 0 or r.sum() == 0 or c.sum() == 0:
+        return np.ones(1, dtype=np.complex128)
+    out = np.zeros(a.shape[1], dtype=np.complex128)
+    _check(
+        lib().pqo_laplace_dd_c128(
+            _ptr(a.view(np.float64), ctypes.c_double), a.shape[0], a.shape[1],
+            _ptr(r, ctypes.c_int), _ptr(c, ctypes.c_int), n_jobs,
+            _ptr(out.view(np.float64), ctypes.c_double),
+        )
+    )
+    return out
+
+
+def ref_laplace(matrix, rows, cols) -> np.ndarray:
+    """``permanent_laplace_cpp<double>`` of the reference, called directly."""
+    a, r, c = _prep(matrix, rows, cols)
+    out = np.zeros(max(a.shape[1], 1), dtype=np.complex128)
+    size = ref_lib().pqref_laplace_c128(
+        _ptr(a.view(np.float64), ctypes.c_double), a.shape[0], a.shape[1],
+        _ptr(r, ctypes.c_int), _ptr(c, ctypes.c_int),
+        _ptr(out.view(np.float64), ctypes.c_double),
+    )
+    _check(min(size, 0))  # the shim returns the result length
+    return out[:size]
 
 
 def gray_states(limits, offset: int = 0, count: Optional[int] = None) -> np.ndarray:

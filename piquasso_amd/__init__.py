@@ -7,10 +7,17 @@ Public entry points
 permanent(matrix, rows, cols)        drop-in for piquasso._math.permanent.permanent
 permanent_ex(matrix, rows, cols, mode) one permanent with the double-double parts
 PermanentBatch / permanent_batch     many uneven permanents in one persistent launch
+permanent_laplace(matrix, rows, cols) drop-in for piquasso._math.permanent.permanent_laplace
+permanent_grad / permanent_grad_batch d perm / d A (grad_perm, any shape)
+torch_permanent(A, rows, cols)       differentiable permanent (torch.autograd.Function)
 """
 
-from ._math.permanent import permanent
+from ._math.permanent import permanent, permanent_laplace
 from .engine import PermanentBatch, permanent_batch, permanent_ex
+from .grad import PermanentFunction, permanent_grad, permanent_grad_batch, torch_permanent
 
-__all__ = ["permanent", "permanent_ex", "PermanentBatch", "permanent_batch"]
+__all__ = [
+    "permanent", "permanent_laplace", "permanent_ex", "PermanentBatch", "permanent_batch",
+    "permanent_grad", "permanent_grad_batch", "PermanentFunction", "torch_permanent",
+]
 __version__ = "0.1.0"

@@ -29,6 +29,7 @@ PQP_FLAG_FP32 = 0x2
 PQP_FLAG_STATIC = 0x4
 PQP_FLAG_EXPAND_COLS = 0x8
 PQP_FLAG_KEEP_COLS = 0x10
+PQP_FLAG_GRADIENT = 0x20
 
 PQP_RUN_EAGER = 0
 PQP_RUN_GRAPH = 1
@@ -69,6 +70,15 @@ SIGNATURES = {
     "pqp_batch_time": (_c.c_int, [_vp, _c.c_int, _c.c_int, _fp]),
     "pqp_batch_stats": (_c.c_int, [_vp, _dp, _i64p, _i64p, _i64p]),
     "pqp_batch_destroy": (None, [_vp]),
+    "pqp_batch_gradients": (_c.c_int, [_vp, _c.POINTER(_dp)]),
+    "pqp_laplace_c128": (
+        _c.c_int, [_dp, _c.c_int64, _c.c_int64, _c.c_int64, _i32p, _i32p, _dp],
+    ),
+    "pqp_grad_batch_c128": (
+        _c.c_int,
+        [_c.c_int64, _c.POINTER(_dp), _i64p, _i64p, _i64p, _c.POINTER(_i32p),
+         _c.POINTER(_i32p), _dp, _c.POINTER(_dp)],
+    ),
     "pqp_plan_image": (
         _c.c_int,
         [_dp, _c.c_int64, _c.c_int64, _c.c_int64, _i32p, _i32p, _c.c_uint32,

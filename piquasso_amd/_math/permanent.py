@@ -27,6 +27,17 @@ Deliberate differences, where the reference has undefined behaviour: a
 ``rows``/``cols`` length that does not match the matrix shape, a non-2-D
 matrix, or a negative multiplicity raise ``ValueError`` instead of reading
 out of bounds.
+
+``permanent_laplace(matrix, rows, cols)`` mirrors the second pybind entry
+(piquasso/_math/permanent.cpp:43-58,79-84; src/permanent_laplace.cpp:43-240):
+element l of the result is perm(A; rows, cols - e_l), for a column with
+cols[l] = 0 the uncut product sum (what the reference's loop produces there),
+and ``[1]`` of length one when the matrix is empty or either multiplicity sum
+is 0 (src/permanent_laplace.cpp:52-57).  Same dtype dispatch as
+``permanent``; the complex64 path is evaluated with FP64 products and
+rounded to complex64 at the end (the reference runs it in FP32).  The
+reference does not check ``sum(cols) == sum(rows) + 1`` and then returns a
+number that is no permanent; here that raises ``RuntimeError``.
 """
 
 from __future__ import annotations
@@ -37,7 +48,7 @@ import numpy as np
 
 from .. import _capi
 
-__all__ = ["permanent", "resolve_dtype"]
+__all__ = ["permanent", "permanent_laplace", "resolve_dtype"]
 
 
 def resolve_dtype(matrix):
@@ -93,3 +104,27 @@ def permanent(matrix, rows, cols):
     )
     _capi.check(st)
     return np.array(complex(out[0], out[1]), dtype=np.complex64)
+
+
+def permanent_laplace(matrix, rows, cols):
+    """All column-minor permanents perm(A; rows, cols - e_l), l = 0..m_c-1,
+    from one Gray sweep per work unit on the GPU (prefix / suffix products,
+    Lemma 1 of arXiv:2005.04214).  Returns a 1-D ``numpy.ndarray``."""
+    dtype = resolve_dtype(matrix)
+    a = np.ascontiguousarray(np.asarray(matrix, dtype=np.complex128))
+    if a.ndim != 2:
+        raise ValueError(f"permanent_laplace(): matrix must be 2-D, got shape {a.shape}")
+    m_r, m_c = a.shape
+    r = _multiplicities(rows, "rows", m_r)
+    c = _multiplicities(cols, "cols", m_c)
+    if m_r == 0 or m_c == 0 or int(r.sum()) == 0 or int(c.sum()) == 0:
+        return np.ones(1, dtype=dtype)
+    out = np.zeros(2 * m_c, dtype=np.float64)
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    dp = ctypes.POINTER(ctypes.c_double)
+    st = _capi.lib().pqp_laplace_c128(
+        a.view(np.float64).ctypes.data_as(dp), m_r, m_c, m_c,
+        r.ctypes.data_as(i32p), c.ctypes.data_as(i32p), out.ctypes.data_as(dp),
+    )
+    _capi.check(st)
+    return (out[0::2] + 1j * out[1::2]).astype(dtype)

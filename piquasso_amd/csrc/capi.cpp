@@ -169,12 +169,12 @@ void index_units(Exec &x) {
 // Upload the plan (allocating device buffers as needed).
 int upload(Exec &x) {
   const Plan &p = x.plan;
-  const size_t n_items = std::max<size_t>(p.items.size(), 1);
+  const size_t n_out = static_cast<size_t>(std::max<int64_t>(p.n_out, 1));
   int st;
   if ((st = x.arena.ensure(p.arena.size())) != PQP_OK) return st;
   if ((st = x.partials.ensure(std::max<int64_t>(p.n_partials, 1) * 4 * sizeof(double))) != PQP_OK) return st;
-  if ((st = x.out.ensure(n_items * 2 * sizeof(double))) != PQP_OK) return st;
-  if ((st = x.out_dd.ensure(n_items * 4 * sizeof(double))) != PQP_OK) return st;
+  if ((st = x.out.ensure(n_out * 2 * sizeof(double))) != PQP_OK) return st;
+  if ((st = x.out_dd.ensure(n_out * 4 * sizeof(double))) != PQP_OK) return st;
   if ((st = x.heads.ensure(64 * sizeof(int))) != PQP_OK) return st;
   PQP_HIP(hipMemcpyAsync(x.arena.p, p.arena.data(), p.arena.size(), hipMemcpyHostToDevice, x.stream));
   PQP_HIP(hipMemsetAsync(x.heads.p, 0, 64 * sizeof(int), x.stream));
@@ -200,6 +200,7 @@ int enqueue(Exec &x, int mode, hipEvent_t ev_begin, hipEvent_t ev_end) {
   const int n_items = static_cast<int>(p.items.size());
   const bool kernel_only = (mode == PQP_RUN_KERNEL);
   if (ev_begin) PQP_HIP(hipEventRecord(ev_begin, x.stream));
+  if (p.minors && mode == PQP_RUN_PLAIN) return fail(PQP_ERR_INVALID, "no plain mode for column minors");
   if (mode == PQP_RUN_PLAIN) {
     for (int k = 0; k < n_items; ++k) {
       const ItemDesc &d = p.items[k];
@@ -250,7 +251,7 @@ int enqueue(Exec &x, int mode, hipEvent_t ev_begin, hipEvent_t ev_end) {
       l.fp32 = p.fp32;
       l.comp = !(x.flags & PQP_FLAG_PLAIN) || p.fp32;
       l.stream = s;
-      if (l.persistent) {
+      if (l.persistent && !p.minors) {
         // resident waves in proportion to the range's share of the work, so
         // concurrent ranges run side by side instead of one after another
         const int per_cu = blocks_per_cu(r.bucket, l.fp32, l.comp, true);
@@ -258,7 +259,10 @@ int enqueue(Exec &x, int mode, hipEvent_t ev_begin, hipEvent_t ev_end) {
         const int slots = static_cast<int>(std::ceil(share * dev.cus * per_cu));
         l.grid = std::max(1, std::min(r.count, std::max(slots, dev.cus / 4)));
       }
-      PQP_HIP(launch_fwd(r.bucket, l));
+      if (p.minors)
+        PQP_HIP(launch_minors(r.bucket, a, r.first, r.count, s));
+      else
+        PQP_HIP(launch_fwd(r.bucket, l));
       if (i > 0) {
         PQP_HIP(hipEventRecord(x.ev_join[i], s));
         PQP_HIP(hipStreamWaitEvent(x.stream, x.ev_join[i], 0));
@@ -266,7 +270,10 @@ int enqueue(Exec &x, int mode, hipEvent_t ev_begin, hipEvent_t ev_end) {
     }
     if (kernel_only && ev_end) PQP_HIP(hipEventRecord(ev_end, x.stream));
     const bool comp = !(x.flags & PQP_FLAG_PLAIN) || p.fp32;
-    PQP_HIP(launch_finalize(comp, a, 0, n_items, static_cast<int>(p.ranges.size()), x.stream));
+    if (p.minors)
+      PQP_HIP(launch_minors_finalize(a, n_items, x.stream));
+    else
+      PQP_HIP(launch_finalize(comp, a, 0, n_items, static_cast<int>(p.ranges.size()), x.stream));
   }
   if (ev_end && !kernel_only) PQP_HIP(hipEventRecord(ev_end, x.stream));
   return PQP_OK;
@@ -276,6 +283,14 @@ int enqueue(Exec &x, int mode, hipEvent_t ev_begin, hipEvent_t ev_end) {
 
 struct pqp_batch {
   Exec x;
+  // gradient batches: the column-minor items (one per row of non-zero
+  // multiplicity of every permanent) run after the permanents on x's stream
+  bool grad = false;
+  Exec xm;
+  std::vector<std::vector<int32_t>> minor_rows;  // r - e_i of every minor item
+  std::vector<int64_t> minor_item, minor_row;    // (permanent, row) of each minor item
+  std::vector<int64_t> m_r, m_c;                 // shapes of the permanents
+  std::vector<std::vector<int32_t>> rows, cols;  // multiplicities of the permanents
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   bool ran = false;
@@ -335,6 +350,74 @@ int run_single(const PermInput &in, uint32_t flags, double *out2, double *out_dd
   return PQP_OK;
 }
 
+// Column-minor inputs of the gradients of `inputs`: one item per row with
+// non-zero multiplicity, rows r - e_i, columns c.
+void minor_inputs(const std::vector<PermInput> &inputs, std::vector<std::vector<int32_t>> &rows,
+                  std::vector<int64_t> &item, std::vector<int64_t> &row,
+                  std::vector<PermInput> &out) {
+  rows.clear();
+  item.clear();
+  row.clear();
+  out.clear();
+  for (size_t k = 0; k < inputs.size(); ++k) {
+    const PermInput &in = inputs[k];
+    for (int64_t i = 0; i < in.m_r; ++i)
+      if (in.rows[i] > 0) {
+        rows.emplace_back(in.rows, in.rows + in.m_r);
+        rows.back()[i] -= 1;
+        item.push_back(static_cast<int64_t>(k));
+        row.push_back(i);
+      }
+  }
+  for (size_t q = 0; q < rows.size(); ++q) {
+    const PermInput &in = inputs[item[q]];
+    out.push_back(PermInput{in.a, in.a_is_float, in.m_r, in.m_c, in.ld, rows[q].data(), in.cols});
+  }
+}
+
+// Complex results of the column-minor plan of x (host copy of x.out).
+int fetch_out(Exec &x, std::vector<double> &res) {
+  res.assign(static_cast<size_t>(std::max<int64_t>(x.plan.n_out, 1)) * 2, 0.0);
+  PQP_HIP(hipMemcpyAsync(res.data(), x.out.p, res.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         x.stream));
+  PQP_HIP(hipStreamSynchronize(x.stream));
+  return PQP_OK;
+}
+
+// grad[i, j] = r_i c_j perm(A; r - e_i, c - e_j) from the minor results.
+void scatter_gradients(const Plan &mp, const std::vector<double> &res,
+                       const std::vector<int64_t> &mitem, const std::vector<int64_t> &mrow,
+                       const std::vector<PermInput> &inputs, double *const *grads) {
+  for (size_t k = 0; k < inputs.size(); ++k)
+    if (grads[k])
+      std::memset(grads[k], 0, static_cast<size_t>(inputs[k].m_r * inputs[k].m_c) * 2 * sizeof(double));
+  for (size_t q = 0; q < mitem.size(); ++q) {
+    const PermInput &in = inputs[mitem[q]];
+    double *g = grads[mitem[q]];
+    if (!g) continue;
+    const ItemInfo &inf = mp.info[q];
+    const int64_t i = mrow[q];
+    const double ri = static_cast<double>(in.rows[i]);
+    for (size_t l = 0; l < inf.colsrc.size(); ++l) {
+      const int64_t j = inf.colsrc[l];
+      const double f = ri * static_cast<double>(in.cols[j]);
+      const double *m = res.data() + 2 * (inf.out_offset + static_cast<int64_t>(l));
+      g[2 * (i * in.m_c + j)] = f * m[0];
+      g[2 * (i * in.m_c + j) + 1] = f * m[1];
+    }
+  }
+}
+
+}  // namespace
+
+namespace {
+// One evaluation of a batch: the permanents, then (gradient batches) the
+// column minors on the same stream.
+int enqueue_all(pqp_batch *b, int mode, hipEvent_t ev_begin, hipEvent_t ev_end) {
+  int st = enqueue(b->x, mode, ev_begin, b->grad ? nullptr : ev_end);
+  if (st != PQP_OK || !b->grad) return st;
+  return enqueue(b->xm, mode == PQP_RUN_PLAIN ? PQP_RUN_EAGER : mode, nullptr, ev_end);
+}
 }  // namespace
 
 extern "C" {
@@ -377,6 +460,106 @@ int pqp_permanent_c64(const float *matrix, int64_t m_r, int64_t m_c, int64_t ld,
   return PQP_OK;
 }
 
+int pqp_laplace_c128(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
+                     const int32_t *rows, const int32_t *cols, double *out) {
+  if (!out && m_c > 0) return fail(PQP_ERR_INVALID, "out is null");
+  PermInput in{matrix, false, m_r, m_c, ld, rows, cols};
+  PlanOptions opt;
+  opt.minors = true;
+  const DeviceState &dev = device_state();
+  opt.min_units = static_cast<int64_t>(std::max(dev.cus, 1)) * kUnitsPerCu;
+  Plan plan;
+  std::string err;
+  int st = plan_build(std::vector<PermInput>{in}, opt, plan, err);
+  if (st != PQP_OK) return fail(st, err);
+  if ((st = require_device()) != PQP_OK) return st;
+  SingleState &S = single_state();
+  std::lock_guard<std::mutex> lock(S.mu);
+  Exec &x = S.x;
+  if (!S.init) {
+    PQP_HIP(hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking));
+    x.own_stream = true;
+    S.init = true;
+  }
+  x.plan = std::move(plan);
+  index_units(x);
+  x.flags = PQP_FLAG_STATIC;
+  if ((st = upload(x)) != PQP_OK) return st;
+  if ((st = enqueue(x, PQP_RUN_EAGER, nullptr, nullptr)) != PQP_OK) return st;
+  std::vector<double> res;
+  if ((st = fetch_out(x, res)) != PQP_OK) return st;
+  const ItemInfo &inf = x.plan.info[0];
+  const int mc = static_cast<int>(inf.colsrc.size());
+  for (int64_t j = 0; j < m_c; ++j) {
+    // columns of zero multiplicity get the uncut product (slot mc), as the
+    // reference's loop does for them (src/permanent_laplace.cpp:168-172)
+    int64_t slot = mc;
+    for (int l = 0; l < mc; ++l)
+      if (inf.colsrc[l] == j) slot = l;
+    out[2 * j] = res[2 * (inf.out_offset + slot)];
+    out[2 * j + 1] = res[2 * (inf.out_offset + slot) + 1];
+  }
+  return PQP_OK;
+}
+
+int pqp_grad_batch_c128(int64_t count, const double *const *matrices, const int64_t *m_r,
+                        const int64_t *m_c, const int64_t *ld, const int32_t *const *rows,
+                        const int32_t *const *cols, double *perms, double *const *grads) {
+  if (count < 0 || (count > 0 && (!matrices || !m_r || !m_c || !ld || !rows || !cols || !grads)))
+    return fail(PQP_ERR_INVALID, "invalid gradient batch arguments");
+  std::vector<PermInput> inputs;
+  for (int64_t k = 0; k < count; ++k)
+    inputs.push_back(PermInput{matrices[k], false, m_r[k], m_c[k], ld[k], rows[k], cols[k]});
+  // validate the permanents themselves (sum check, shapes) before the minors
+  Plan fwd;
+  std::string err;
+  PlanOptions fopt;
+  fopt.total_units = static_cast<int64_t>(std::max(device_state().cus, 1)) * kUnitsPerCu;
+  int st = plan_build(inputs, fopt, fwd, err);
+  if (st != PQP_OK) return fail(st, err);
+  std::vector<std::vector<int32_t>> mrows;
+  std::vector<int64_t> mitem, mrow;
+  std::vector<PermInput> minputs;
+  minor_inputs(inputs, mrows, mitem, mrow, minputs);
+  PlanOptions mopt;
+  mopt.minors = true;
+  mopt.total_units = fopt.total_units;
+  Plan mplan;
+  st = plan_build(minputs, mopt, mplan, err);
+  if (st != PQP_OK) return fail(st, err);
+  if ((st = require_device()) != PQP_OK) return st;
+  SingleState &S = single_state();
+  std::lock_guard<std::mutex> lock(S.mu);
+  Exec &x = S.x;
+  if (!S.init) {
+    PQP_HIP(hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking));
+    x.own_stream = true;
+    S.init = true;
+  }
+  std::vector<double> res;
+  if (perms && count > 0) {
+    x.plan = std::move(fwd);
+    index_units(x);
+    x.flags = PQP_FLAG_STATIC;
+    if ((st = upload(x)) != PQP_OK) return st;
+    if ((st = enqueue(x, PQP_RUN_EAGER, nullptr, nullptr)) != PQP_OK) return st;
+    if ((st = fetch_out(x, res)) != PQP_OK) return st;
+    std::memcpy(perms, res.data(), static_cast<size_t>(count) * 2 * sizeof(double));
+  }
+  if (!minputs.empty()) {
+    x.plan = std::move(mplan);
+    index_units(x);
+    x.flags = PQP_FLAG_STATIC;
+    if ((st = upload(x)) != PQP_OK) return st;
+    if ((st = enqueue(x, PQP_RUN_EAGER, nullptr, nullptr)) != PQP_OK) return st;
+    if ((st = fetch_out(x, res)) != PQP_OK) return st;
+    scatter_gradients(x.plan, res, mitem, mrow, inputs, grads);
+  } else {
+    scatter_gradients(x.plan, res, mitem, mrow, inputs, grads);
+  }
+  return PQP_OK;
+}
+
 int pqp_batch_create(int64_t count, const void *const *matrices, int is_c64, const int64_t *m_r,
                      const int64_t *m_c, const int64_t *ld, const int32_t *const *rows,
                      const int32_t *const *cols, uint32_t flags, int64_t target_inner,
@@ -402,6 +585,25 @@ int pqp_batch_create(int64_t count, const void *const *matrices, int is_c64, con
   std::string err;
   int st = plan_build(inputs, opt, b->x.plan, err);
   if (st != PQP_OK) return fail(st, err);
+  if (flags & PQP_FLAG_GRADIENT) {
+    if (is_c64) return fail(PQP_ERR_INVALID, "gradients need complex128 input");
+    b->grad = true;
+    for (int64_t k = 0; k < count; ++k) {
+      b->m_r.push_back(m_r[k]);
+      b->m_c.push_back(m_c[k]);
+      b->rows.emplace_back(rows[k], rows[k] + m_r[k]);
+      b->cols.emplace_back(cols[k], cols[k] + m_c[k]);
+      inputs[k].rows = b->rows.back().data();
+      inputs[k].cols = b->cols.back().data();
+    }
+    std::vector<PermInput> minputs;
+    minor_inputs(inputs, b->minor_rows, b->minor_item, b->minor_row, minputs);
+    PlanOptions mopt;
+    mopt.minors = true;
+    mopt.total_units = opt.total_units;
+    st = plan_build(minputs, mopt, b->xm.plan, err);
+    if (st != PQP_OK) return fail(st, err);
+  }
   if ((st = require_device()) != PQP_OK) return st;
   index_units(b->x);
   b->x.flags = flags;
@@ -410,6 +612,16 @@ int pqp_batch_create(int64_t count, const void *const *matrices, int is_c64, con
   if ((st = upload(b->x)) != PQP_OK) {
     pqp_batch_destroy(b.release());
     return st;
+  }
+  if (b->grad) {
+    index_units(b->xm);
+    b->xm.flags = PQP_FLAG_STATIC;
+    b->xm.stream = b->x.stream;  // shares the batch stream (and its graph)
+    b->xm.own_stream = false;
+    if ((st = upload(b->xm)) != PQP_OK) {
+      pqp_batch_destroy(b.release());
+      return st;
+    }
   }
   PQP_HIP(hipStreamSynchronize(b->x.stream));
   *out = b.release();
@@ -422,7 +634,7 @@ int pqp_batch_run(pqp_batch *b, int mode) {
   if (mode == PQP_RUN_GRAPH) {
     if (!b->graph_exec) {
       PQP_HIP(hipStreamBeginCapture(b->x.stream, hipStreamCaptureModeThreadLocal));
-      st = enqueue(b->x, PQP_RUN_EAGER, nullptr, nullptr);
+      st = enqueue_all(b, PQP_RUN_EAGER, nullptr, nullptr);
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(b->x.stream, &g);
       if (st != PQP_OK) {
@@ -435,7 +647,7 @@ int pqp_batch_run(pqp_batch *b, int mode) {
     }
     PQP_HIP(hipGraphLaunch(b->graph_exec, b->x.stream));
   } else if (mode == PQP_RUN_EAGER || mode == PQP_RUN_PLAIN || mode == PQP_RUN_KERNEL) {
-    if ((st = enqueue(b->x, mode, nullptr, nullptr)) != PQP_OK) return st;
+    if ((st = enqueue_all(b, mode, nullptr, nullptr)) != PQP_OK) return st;
   } else {
     return fail(PQP_ERR_INVALID, "unknown run mode");
   }
@@ -475,7 +687,7 @@ int pqp_batch_time(pqp_batch *b, int mode, int reps, float *ms) {
       (void)hipGraphLaunch(b->graph_exec, b->x.stream);
       (void)hipEventRecord(e1, b->x.stream);
     } else {
-      st = enqueue(b->x, mode, e0, e1);
+      st = enqueue_all(b, mode, e0, e1);
       if (st != PQP_OK) break;
     }
     hipError_t e = hipEventSynchronize(e1);
@@ -494,10 +706,28 @@ int pqp_batch_time(pqp_batch *b, int mode, int reps, float *ms) {
 int pqp_batch_stats(const pqp_batch *b, double *flops, int64_t *terms, int64_t *units,
                     int64_t *arena_bytes) {
   if (!b) return fail(PQP_ERR_INVALID, "batch is null");
-  if (flops) *flops = b->x.plan.total_flops;
-  if (terms) *terms = b->x.plan.total_terms;
-  if (units) *units = static_cast<int64_t>(b->x.plan.units.size());
-  if (arena_bytes) *arena_bytes = static_cast<int64_t>(b->x.plan.arena.size());
+  if (flops) *flops = b->x.plan.total_flops + (b->grad ? b->xm.plan.total_flops : 0.0);
+  if (terms) *terms = b->x.plan.total_terms + (b->grad ? b->xm.plan.total_terms : 0);
+  if (units)
+    *units = static_cast<int64_t>(b->x.plan.units.size() + (b->grad ? b->xm.plan.units.size() : 0));
+  if (arena_bytes)
+    *arena_bytes =
+        static_cast<int64_t>(b->x.plan.arena.size() + (b->grad ? b->xm.plan.arena.size() : 0));
+  return PQP_OK;
+}
+
+int pqp_batch_gradients(pqp_batch *b, double *const *grads) {
+  if (!b || !grads) return fail(PQP_ERR_INVALID, "invalid arguments");
+  if (!b->grad) return fail(PQP_ERR_INVALID, "batch was not created with PQP_FLAG_GRADIENT");
+  if (!b->ran) return fail(PQP_ERR_INVALID, "batch has not been run");
+  std::vector<double> res;
+  int st;
+  if (!b->minor_item.empty() && (st = fetch_out(b->xm, res)) != PQP_OK) return st;
+  std::vector<PermInput> inputs;
+  for (size_t k = 0; k < b->m_r.size(); ++k)
+    inputs.push_back(PermInput{nullptr, false, b->m_r[k], b->m_c[k], b->m_c[k],
+                               b->rows[k].data(), b->cols[k].data()});
+  scatter_gradients(b->xm.plan, res, b->minor_item, b->minor_row, inputs, grads);
   return PQP_OK;
 }
 
@@ -511,6 +741,7 @@ int pqp_plan_image(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
   opt.expand_cols = expand_mode(flags);
   opt.target_inner = target_inner;
   opt.min_units = min_units;
+  opt.minors = (flags & PQP_FLAG_GRADIENT) != 0;
   Plan p;
   std::string err;
   const int st = plan_build(std::vector<PermInput>{in}, opt, p, err);
@@ -528,6 +759,16 @@ void pqp_batch_destroy(pqp_batch *b) {
   if (b->x.stream) (void)hipStreamSynchronize(b->x.stream);
   if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->xm.arena.release();
+  b->xm.partials.release();
+  b->xm.out.release();
+  b->xm.out_dd.release();
+  b->xm.heads.release();
+  for (int i = 1; i < kBuckets; ++i) {
+    if (b->xm.side[i]) (void)hipStreamDestroy(b->xm.side[i]);
+    if (b->xm.ev_join[i]) (void)hipEventDestroy(b->xm.ev_join[i]);
+  }
+  if (b->xm.ev_fork) (void)hipEventDestroy(b->xm.ev_fork);
   b->x.arena.release();
   b->x.partials.release();
   b->x.out.release();

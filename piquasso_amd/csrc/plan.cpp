@@ -111,6 +111,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
   plan.items_off = arena_alloc(plan.arena, n_items * int64_t(sizeof(ItemDesc)));
 
   std::map<std::vector<int>, InnerTables> inner_cache;
+  int64_t out_cursor = 0;
   std::vector<int64_t> item_units(n_items, 0);
   std::vector<int> item_bucket(n_items, -1);
   int64_t partial_cursor = 0;
@@ -134,7 +135,8 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
       sc += std::max(in.cols[j], 0);
       mc += in.cols[j] > 0;
     }
-    if (sr == 0 || sr != sc || sr > 64) continue;  // reported in pass 2
+    const int64_t extra = opt.minors ? 1 : 0;  // minors: sum(c) = sum(r) + 1
+    if (sr == 0 || sr + extra != sc || sc > 64) continue;  // reported in pass 2
     // T = prod (rho + 1) with the prepend applied to the first minimal row
     int minelem = 0;
     int64_t i0 = 0;
@@ -148,7 +150,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
       const int rho = in.rows[i] - (i == i0 ? 1 : 0);
       if (rho > 0) T *= rho + 1;
     }
-    item_cost[k] = T * (2.0 * mc + 4.0 * (sr - 1) + 8.0);
+    item_cost[k] = opt.minors ? T * (16.0 * mc + 10.0) : T * (2.0 * mc + 4.0 * (sr - 1) + 8.0);
     total_cost += item_cost[k];
   }
 
@@ -157,7 +159,8 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     const PermInput &in = inputs[k];
     ItemDesc d;
     std::memset(&d, 0, sizeof(d));
-    d.out_index = static_cast<int32_t>(k);
+    d.out_index = static_cast<int32_t>(out_cursor);
+    plan.info[k].out_offset = out_cursor;
     int64_t item_min_units = opt.min_units;
     if (opt.total_units > 0 && total_cost > 0.0) {
       const double share = item_cost[k] / total_cost * static_cast<double>(opt.total_units);
@@ -184,20 +187,35 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
       }
       sc += in.cols[j];
     }
-    if (sr != sc) {
+    if (!opt.minors && sr != sc) {
       err = "Number of input and output states should be equal";
       return kErrSumMismatch;
     }
+    if (opt.minors && sr + 1 != sc) {
+      err = "column multiplicities must sum to one more than the row multiplicities";
+      return kErrSumMismatch;
+    }
     if (sr == 0) {
-      // perm of an empty selection is 1 (src/permanent.cpp:106-108)
+      // perm of an empty selection is 1 (src/permanent.cpp:106-108); a
+      // column-minor item of an empty row set has the minor 1 in its one
+      // column of multiplicity 1
       plan.info[k].trivial_one = true;
       d.units = 0;
       d.n = 0;
       d.partial_base = partial_cursor;
+      if (opt.minors) {
+        d.flags = kMinors;
+        for (int64_t j = 0; j < in.m_c; ++j)
+          if (in.cols[j] > 0) plan.info[k].colsrc.push_back(j);
+        d.mc = static_cast<int32_t>(plan.info[k].colsrc.size());
+        out_cursor += d.mc + 1;
+      } else {
+        out_cursor += 1;
+      }
       plan.items[k] = d;
       continue;
     }
-    if (sr > 64) {
+    if (sc > 64) {
       err = "item " + std::to_string(k) + ": more than 64 particles";
       return kErrRange;
     }
@@ -218,7 +236,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     // wave-uniform branches), or is expanded into c identical columns of
     // multiplicity 1 (branch-free product, 2 (c - 1) more additions per term).
     bool expand = false;
-    if (cmax_in > 1) {
+    if (cmax_in > 1 && !opt.minors) {
       if (opt.expand_cols == 1) expand = n <= kMaxCols;
       if (opt.expand_cols == 2) expand = n <= kExpandAutoMax;
     }
@@ -238,6 +256,10 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     const int mc = static_cast<int>(colsrc.size());
     int mcp = 0;
     const int bucket = bucket_for_width(mc, opt.fp32_products, &mcp);
+    if (opt.minors && mcp > 32) {
+      err = "item " + std::to_string(k) + ": column minors support at most 32 columns";
+      return kErrRange;
+    }
     if (bucket < 0) {
       err = "item " + std::to_string(k) + ": " + std::to_string(mc) +
             " columns exceed the widest kernel (64)";
@@ -280,8 +302,12 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     });
     int cmax = 0;
     for (int c : colmult) cmax = std::max(cmax, c);
-    const double step_cost = 2.0 * mc + 4.0 * (n - 1) + 2.0 + 6.0 +
-                             (cmax > 1 ? 2.0 * (n - mc) : 0.0);
+    const double step_cost =
+        opt.minors ? 16.0 * mc + 10.0 + (cmax > 1 ? 8.0 * (n + 1 - mc) : 0.0)
+                   : 2.0 * mc + 4.0 * (n - 1) + 2.0 + 6.0 + (cmax > 1 ? 2.0 * (n - mc) : 0.0);
+    // per-unit overhead: start state, plus the wave reduction of every
+    // column accumulator for column-minor items
+    const double unit_extra = opt.minors ? 160.0 * (mc + 1) : 0.0;
     int best_L = 0;
     double best_total = -1.0;
     {
@@ -292,7 +318,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
         const int64_t U = T / S;
         const int64_t units = (U + kWave - 1) / kWave;
         const int Dt = D - L;
-        const double init_cost = (1.0 + D) * 2.0 * mcp + Dt * 40.0 + 400.0;
+        const double init_cost = (1.0 + D) * 2.0 * mcp + Dt * 40.0 + 400.0 + unit_extra;
         const double unit_cost = init_cost + static_cast<double>(S) * step_cost;
         const double total = static_cast<double>(units) * unit_cost;
         bool ok = true;
@@ -432,6 +458,8 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
       }
     }
     // column multiplicity counts: cpow[r-2] = #{j : c_j >= r}, r = 2..cmax
+    d.cmul_off = arena_alloc(plan.arena, int64_t(mcp) * 4);
+    for (int j = 0; j < mc; ++j) at<int32_t>(plan.arena, d.cmul_off)[j] = colmult[j];
     d.cpow_off = arena_alloc(plan.arena, int64_t(std::max(cmax - 1, 1)) * 4);
     for (int r = 2; r <= cmax; ++r) {
       int cnt = 0;
@@ -448,14 +476,23 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     d.L = L;
     d.Dt = Dt;
     d.cmax = cmax;
-    d.flags = (cmax == 1 ? kColsAllOnes : 0) | (top_binary ? kTopBinary : 0);
+    d.flags = (cmax == 1 ? kColsAllOnes : 0) | (top_binary ? kTopBinary : 0) |
+              (opt.minors ? kMinors : 0);
     d.partial_base = partial_cursor;
-    partial_cursor += d.units;
+    partial_cursor += static_cast<int64_t>(d.units) * (opt.minors ? mcp + 1 : 1);
+    out_cursor += opt.minors ? mc + 1 : 1;
+    plan.info[k].colsrc.assign(colsrc.begin(), colsrc.end());
     plan.items[k] = d;
 
     ItemInfo &inf = plan.info[k];
     inf.terms = T;
-    inf.flops = static_cast<double>(T) * (2.0 * mc_alg + 6.0 * (n - 1) + 4.0);
+    // minors: row update, prefix / suffix / combine products (3 (m_c - 1)
+    // complex multiplies, extra powers 2 (n + 1 - m_c)), m_c + 1 accumulations
+    inf.flops = opt.minors
+                    ? static_cast<double>(T) * (2.0 * mc_alg + 6.0 * (3.0 * (mc_alg - 1) +
+                                                                   2.0 * (n + 1 - mc_alg)) +
+                                                4.0 * (mc_alg + 1))
+                    : static_cast<double>(T) * (2.0 * mc_alg + 6.0 * (n - 1) + 4.0);
     inf.unit_cost = ((1.0 + D) * 2.0 * mcp + Dt * 40.0 + 400.0) +
                     static_cast<double>(S_in) * step_cost;
     plan.total_flops += inf.flops;
@@ -464,6 +501,8 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     item_bucket[k] = bucket;
   }
   plan.n_partials = partial_cursor;
+  plan.n_out = out_cursor;
+  plan.minors = opt.minors;
 
   // --- work units: grouped by kernel bucket, LPT (costliest units first) ---
   std::vector<int64_t> idx(n_items);

@@ -43,6 +43,9 @@ struct PlanOptions {
   int expand_cols = 2;          // columns with multiplicity c > 1: 0 keep (power c),
                                 // 1 expand into c columns, 2 planner's choice (keep:
                                 // expansion widens the kernel and lowers occupancy)
+  bool minors = false;          // column-minor items: sum(c) = sum(r) + 1, outputs
+                                // perm(A; r, c - e_l) for every column l plus the
+                                // uncut product (m_c + 1 values per item)
   int64_t total_units = 0;      // spread about this many waves over the batch, in
                                 // proportion to each permanent's cost (0 = off)
 };
@@ -53,6 +56,8 @@ struct ItemInfo {
   double flops = 0.0;           // T * (2 m_c + 6 (n - 1) + 4)
   double unit_cost = 0.0;       // modelled cost of one wave unit (LPT key)
   bool trivial_one = false;     // n == 0: permanent is 1 by definition
+  int64_t out_offset = 0;       // first output slot (complex values)
+  std::vector<int64_t> colsrc;  // source column of each kernel column
 };
 
 struct Plan {
@@ -66,7 +71,9 @@ struct Plan {
   // contiguous unit ranges per kernel bucket: {bucket id, first unit, count}
   struct Range { int bucket; int32_t first; int32_t count; double cost; };
   std::vector<Range> ranges;
-  int64_t n_partials = 0;
+  int64_t n_partials = 0;     // partial slots (4 doubles each)
+  int64_t n_out = 0;          // output slots (complex values)
+  bool minors = false;        // column-minor items (perm_minors.hip kernels)
   bool fp32 = false;
   double total_flops = 0.0;
   int64_t total_terms = 0;

@@ -86,6 +86,10 @@ class PermanentBatch:
     total_units:
         wave work units spread over the batch in proportion to each
         permanent's cost (0: 48 per compute unit).
+    gradient:
+        also evaluate d perm / d A of every item on each run (column-minor
+        sweeps after the permanents, on the same stream and in the same
+        graph); read them with :meth:`gradients`.  complex128 / "fp64" only.
     """
 
     def __init__(
@@ -98,6 +102,7 @@ class PermanentBatch:
         target_inner: int = 0,
         expand_cols=None,
         total_units: int = 0,
+        gradient: bool = False,
     ):
         if not (len(matrices) == len(rows) == len(cols)):
             raise ValueError("matrices, rows and cols must have the same length")
@@ -105,6 +110,7 @@ class PermanentBatch:
             raise ValueError("precision must be 'fp64' or 'fp32'")
         self.count = len(matrices)
         self.precision = precision
+        self.gradient = bool(gradient)
         self._keep = []
         mats, rws, cls = [], [], []
         for m, r, c in zip(matrices, rows, cols):
@@ -128,6 +134,8 @@ class PermanentBatch:
             flags |= _capi.PQP_FLAG_EXPAND_COLS
         elif expand_cols is False:
             flags |= _capi.PQP_FLAG_KEEP_COLS
+        if gradient:
+            flags |= _capi.PQP_FLAG_GRADIENT
         handle = ctypes.c_void_p()
         L = _capi.lib()
         st = L.pqp_batch_create(
@@ -138,6 +146,7 @@ class PermanentBatch:
         )
         _capi.check(st)
         self._h = handle
+        self._shapes = [(int(m_r[k]), int(m_c[k])) for k in range(n)]
         self._keep = None  # inputs were copied into the plan
 
     # -- running ---------------------------------------------------------
@@ -164,6 +173,14 @@ class PermanentBatch:
         )
         vals = (out[:, 0] + 1j * out[:, 1])[: self.count]
         return (vals, dd[: self.count]) if with_dd else vals
+
+    def gradients(self):
+        """d perm / d A of every item from the last run (gradient batches):
+        a list of complex128 arrays of the items' shapes."""
+        grads = [np.zeros(sh, dtype=np.complex128) for sh in self._shapes]
+        ptrs = (_dp * max(self.count, 1))(*[g.view(np.float64).ctypes.data_as(_dp) for g in grads])
+        _capi.check(_capi.lib().pqp_batch_gradients(self._h, ptrs))
+        return grads
 
     def time(self, mode: str = "graph", reps: int = 5) -> np.ndarray:
         """Milliseconds per evaluation for `reps` evaluations, timed with HIP

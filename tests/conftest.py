@@ -1,4 +1,5 @@
 import json
+import math
 import os
 import sys
 
@@ -63,3 +64,35 @@ def rel_err(x, ref):
     ref = complex(ref)
     den = abs(ref)
     return abs(x - ref) / den if den > 0 else abs(x - ref)
+
+
+def glynn_abs_sum(a, rows, cols):
+    """sum_k |term_k| / 2^(n-1) of the Glynn sum with the reference's prepend
+    rule (first minimal non-zero row fixed at +1, src/permanent.cpp:54-95).
+    Rounding bounds scale with it: |computed - exact| <~ (n + 2) eps times
+    this sum, however much the terms cancel.  The column multiplicities need
+    not sum to the row multiplicities (column-minor products)."""
+    import itertools
+
+    a = np.asarray(a, dtype=np.complex128)
+    rows = [int(x) for x in np.asarray(rows).reshape(-1)]
+    cols = np.asarray(cols, dtype=np.int64).reshape(-1)
+    n = sum(rows)
+    if n == 0:
+        return 1.0
+    nz = [x for x in rows if x > 0]
+    i0 = rows.index(min(nz))
+    rho = list(rows)
+    rho[i0] -= 1
+    live = [i for i, x in enumerate(rho) if x > 0]
+    if live:
+        K = np.array(list(itertools.product(*[range(rho[i] + 1) for i in live])), dtype=np.int64)
+    else:
+        K = np.zeros((1, 0), dtype=np.int64)
+    rv = np.array([rho[i] for i in live], dtype=np.int64)
+    w = np.ones(K.shape[0])
+    for t, x in enumerate(rv):
+        w *= np.array([math.comb(int(x), int(k)) for k in range(x + 1)])[K[:, t]]
+    cs = a[i0][None, :] + (rv[None, :] - 2 * K).astype(np.float64) @ a[live] if live else a[i0][None, :]
+    mag = np.prod(np.abs(cs) ** cols[None, :], axis=1)
+    return float(np.sum(w * mag)) / 2.0 ** (n - 1)

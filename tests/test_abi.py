@@ -1,0 +1,96 @@
+"""The C-ABI boundary without a GPU: the library loads, exports every entry
+point that include/pqp_permanent.h declares, the ctypes table binds each of
+them, and the host-side argument handling (dtype dispatch, validation,
+error mapping) behaves like the reference's pybind module.  No compute call
+reaches a device here."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from piquasso_amd import _capi
+from piquasso_amd._math.permanent import resolve_dtype
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pqp_permanent.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pqp_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_entry_points():
+    names = declared_functions()
+    for must in ("pqp_permanent_c128", "pqp_permanent_c64", "pqp_laplace_c128",
+                 "pqp_grad_batch_c128", "pqp_batch_create", "pqp_batch_run",
+                 "pqp_batch_gradients", "pqp_last_error"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_capi.LIB_PATH)
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_ctypes_table_covers_the_header():
+    assert set(declared_functions()) == set(_capi.SIGNATURES)
+
+
+def test_version_and_device_query_without_gpu():
+    L = _capi.lib()
+    assert L.pqp_version().decode().startswith("pqp-permanent")
+    n = ctypes.c_int(-1)
+    st = L.pqp_device_info(ctypes.byref(n), None, None, 0)
+    if st != 0:  # no device in this container: a clean error, not a crash
+        assert st == _capi.PQP_ERR_DEVICE
+        assert L.pqp_last_error().decode()
+
+
+@pytest.mark.parametrize("dtype,expect", [
+    (np.complex64, np.complex64), (np.float32, np.complex64), (np.int8, np.complex64),
+    (np.bool_, np.complex64), (np.complex128, np.complex128), (np.float64, np.complex128),
+    (np.int64, np.complex128), (np.int32, np.complex128),
+])
+def test_dtype_dispatch_mirrors_pybind(dtype, expect):
+    assert resolve_dtype(np.zeros((2, 2), dtype=dtype)) == expect
+
+
+def test_dtype_dispatch_lists_and_errors():
+    assert resolve_dtype([[1.0, 2.0], [3.0, 4.0]]) == np.complex64
+    assert resolve_dtype(np.zeros((2, 4), dtype=np.complex128)[:, ::2]) == np.complex128
+    with pytest.raises(TypeError):
+        resolve_dtype(np.zeros((2, 2), dtype=np.longdouble))
+    with pytest.raises(TypeError):
+        resolve_dtype(np.zeros((2, 2), dtype=object))
+
+
+def test_argument_errors_precede_the_device():
+    """Sum mismatch -> RuntimeError (the reference's message), bad shapes ->
+    ValueError; both raised by the planner before any device is touched."""
+    import piquasso_amd as pa
+
+    with pytest.raises(RuntimeError, match="Number of input and output states"):
+        pa.permanent(np.eye(2, dtype=np.complex128), [1, 1], [2, 1])
+    with pytest.raises(ValueError):
+        pa.permanent(np.eye(2, dtype=np.complex128), [1, 1, 0], [1, 1])
+    with pytest.raises(ValueError):
+        pa.permanent(np.ones(3, dtype=np.complex128), [1], [1])
+    with pytest.raises(ValueError):
+        pa.permanent(np.eye(2, dtype=np.complex128), [1, -1], [0, 0])
+    with pytest.raises(RuntimeError):
+        pa.permanent_laplace(np.eye(2), [1, 1], [1, 1])
+
+
+def test_empty_selection_needs_no_device():
+    """perm of an empty selection is 1 (src/permanent.cpp:106-108); the
+    Laplace entry returns the reference's [1] for degenerate inputs."""
+    import piquasso_amd as pa
+
+    assert pa.permanent_laplace(np.ones((2, 2)), [0, 0], [0, 1]).tolist() == [1]
+    assert pa.permanent_laplace(np.zeros((0, 3)), [], [1, 0, 0]).tolist() == [1]

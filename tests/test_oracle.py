@@ -334,3 +334,68 @@ def test_planted_mistakes_are_caught(oracle):
     assert rel_err(good, oracle.perm_bruteforce(asym, [2, 1], [1, 1, 1])) < 1e-13
     with pytest.raises(ValueError):
         oracle.perm_dd(asym.T, [2, 1], [1, 1, 1])
+
+
+# ------------------------------------------------------- column minors --
+
+
+def _laplace_case(case):
+    a = complex_matrix(case["matrix"]).reshape(case["shape"])
+    return a, case["rows"], case["cols"], complex_matrix(case["reference"]), complex_matrix(case["dd"])
+
+
+def test_laplace_dd_matches_reference_fixtures(oracle):
+    """laplace_dd against outputs of the reference's permanent_laplace_cpp
+    (compiled from its sources) stored in tests/golden/laplace_vectors.json,
+    including zero-multiplicity columns and the degenerate [1] cases."""
+    from conftest import load_json
+
+    for case in load_json("laplace_vectors.json")["cases"]:
+        a, r, c, ref, dd = _laplace_case(case)
+        got = oracle.laplace_dd(a, r, c)
+        assert got.shape == ref.shape, case["id"]
+        scale = max(np.abs(ref).max(), 1e-300)
+        assert np.abs(got - ref).max() <= 1e-10 * scale, case["id"]
+        assert np.abs(got - dd).max() <= 1e-15 * scale, case["id"]
+
+
+def test_laplace_dd_matches_reference_build(oracle):
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref not built (make -C oracle ref)")
+    rng = np.random.default_rng(8)
+    for t in range(60):
+        m_r, m_c = (int(x) for x in rng.integers(1, 6, size=2))
+        n = int(rng.integers(1, 7))
+        a = rng.standard_normal((m_r, m_c)) + 1j * rng.standard_normal((m_r, m_c))
+        r = np.bincount(rng.integers(0, m_r, n), minlength=m_r)
+        c = np.bincount(rng.integers(0, m_c, n + 1), minlength=m_c)
+        x, y = oracle.laplace_dd(a, r, c), oracle.ref_laplace(a, r, c)
+        assert np.abs(x - y).max() <= 1e-11 * max(1.0, np.abs(x).max()), t
+
+
+def test_laplace_dd_exact_minors_and_expansion(oracle):
+    """Each column minor equals the exact rational brute-force permanent of
+    A with that column's multiplicity lowered by one, and the minors expand
+    back to the permanent along any row x: perm(A; r, c) =
+    sum_l c_l a_xl perm(A; r - e_x, c - e_l)."""
+    rng = np.random.default_rng(9)
+    for t in range(12):
+        m_r, m_c = (int(x) for x in rng.integers(2, 5, size=2))
+        n = int(rng.integers(2, 6))
+        a = rng.integers(-3, 4, size=(m_r, m_c)) + 1j * rng.integers(-3, 4, size=(m_r, m_c))
+        r = np.bincount(rng.integers(0, m_r, n), minlength=m_r)
+        c = np.bincount(rng.integers(0, m_c, n), minlength=m_c)
+        full = complex(*(float(v) for v in oracle.perm_bruteforce(a, r, c, exact=True)))
+        for x in np.flatnonzero(r):
+            rx = r.copy()
+            rx[x] -= 1
+            if rx.sum() == 0:
+                continue
+            lap = oracle.laplace_dd(a, rx, c)
+            for l in np.flatnonzero(c):
+                cl = c.copy()
+                cl[l] -= 1
+                er, ei = oracle.perm_bruteforce(a, rx, cl, exact=True)
+                # integer entries: every term and partial sum is exact
+                assert lap[l] == complex(float(er), float(ei)), (t, x, l)
+            assert np.sum(c * a[x] * lap) == full, (t, x)
