@@ -3,8 +3,8 @@
 Headline workload (one "step" = one evaluation of the whole batch): 4096
 permanents of uneven size, n ~ U{8..24}, each a row/column selection with
 multiplicities 1..3 of a 48-mode Haar unitary, complex128, double-double
-accumulation, evaluated by one persistent work-stealing launch per
-column-width bucket replayed from a hipGraph.  Inputs are resident in device
+accumulation, evaluated by one launch per column-width bucket (the buckets
+run concurrently on forked streams) replayed from a hipGraph.  Inputs are resident in device
 memory before the timed region (planning and upload happen once, outside).
 
 Printed as ONE JSON line (rank 0):
@@ -12,8 +12,11 @@ Printed as ONE JSON line (rank 0):
   plain           the same batch through the plain FP64 kernel, one launch per
                   permanent (speedup_vs_plain = value / plain rate)
   n32             one n = 32 permanent (2^31 Gray terms): time, % of FP64 peak
+  gradient        gradients/s of 256 permanents n ~ U{12..20} (d perm / dA)
   rel_err_max     vs the double-double oracle values committed with the inputs
-  roofline        the sweep kernel: algorithmic FLOP / kernel time vs 78.6 TFLOP/s
+  roofline        the sweep kernels: algorithmic FLOP / kernel time vs 78.6 TFLOP/s;
+                  traffic = HBM bytes per evaluation from the committed rocprofv3
+                  FETCH_SIZE + WRITE_SIZE summary in profiles/
   cpu_baseline    the reference's own C++ (oracle/_ref) on the host cores, a
                   bounded sample of the same batch
 
@@ -33,6 +36,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (and matrix) peak, AMD specification
+PROFILES = os.path.join(ROOT, "profiles")
+TRAFFIC_FILE = os.path.join(PROFILES, "hbm_traffic_batched_uneven.txt")
 METRIC = "permanents/s batched n\u2208[8,24] and n=32 %FP64 peak, vs plain, rel.err\u22641e-10"
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
@@ -120,6 +125,48 @@ def run_cpu_baseline(seconds: float):
 
 # ------------------------------------------------------------ GPU leg --
 
+def _traffic_from_profile():
+    """HBM bytes per evaluation of the batch, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel), or
+    None when that summary is absent."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return float(line.split("=")[1].split()[0])
+    except OSError:
+        pass
+    return None
+
+
+def _gradient_leg(pa, W, reps: int):
+    """Gradients of 256 permanents, n ~ U{12..20}, unit multiplicities, Haar
+    48-mode submatrices: forward sweep + one column-minor sweep per row, all
+    in one hipGraph.  Euler's identity sum_ij A_ij G_ij = n perm(A) checks
+    every gradient (relative to sum_ij |A_ij G_ij|)."""
+    mats, rows, cols = W.gradient_batch(count=256, seed=5)
+    with pa.PermanentBatch(mats, rows, cols, gradient=True) as gb:
+        gb.run("graph")
+        gb.sync()
+        perms = gb.results()
+        grads = gb.gradients()
+        ms = gb.time("graph", reps=reps)
+        st = gb.stats()
+    resid = 0.0
+    for a, r, p, g in zip(mats, rows, perms, grads):
+        n = int(np.sum(r))
+        resid = max(resid, abs(np.sum(a * g) - n * p) / np.sum(np.abs(a * g)))
+    med = float(np.median(ms))
+    return {
+        "gradients_per_s": len(mats) / (med * 1e-3),
+        "ms": med,
+        "tflops": st["flops"] / (med * 1e-3) / 1e12,
+        "euler_residual_max": float(resid),
+        "what": "256 gradients d perm/dA (n~U{12..20}, 48-mode Haar submatrices): permanent "
+                "sweep + one column-minor sweep per row, one hipGraph replay",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,7 +175,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-child", type=float, default=None)
-    ap.add_argument("--profile", choices=["batch", "n32"], default=None,
+    ap.add_argument("--profile", choices=["batch", "n32", "grad"], default=None,
                     help="profiling runs: only launch this workload's kernels --steps times")
     ap.add_argument("--copy-calibration", type=int, default=None, metavar="MIB",
                     help="profiling runs: one device-to-device copy of MIB MiB (torch), to "
@@ -155,9 +202,17 @@ def main():
         if args.profile == "batch":
             mats, rows, cols, _ = _load_batched()
             b = pa.PermanentBatch(mats, rows, cols)
+        elif args.profile == "grad":
+            from piquasso_amd import workloads as W
+
+            mats, rows, cols = W.gradient_batch(count=256, seed=5)
+            b = pa.PermanentBatch(mats, rows, cols, gradient=True)
         else:
             a32, r32, c32, _ = _load_n32()
-            b = pa.PermanentBatch([a32], [r32], [c32], persistent=False)
+            from piquasso_amd import _capi
+
+            cus = _capi.device_info()[1]
+            b = pa.PermanentBatch([a32], [r32], [c32], persistent=True, total_units=64 * cus)
         ms = b.time("eager", reps=args.steps)
         print(json.dumps({"profile": args.profile, "ms": ms.tolist()}))
         return 0
@@ -197,7 +252,9 @@ def main():
 
     # single large permanent, n = 32 (2^31 Gray terms)
     a32, r32, c32, ref32 = _load_n32()
-    b32 = pa.PermanentBatch([a32], [r32], [c32], persistent=False)
+    # one large permanent: 64 work units per CU and the work-stealing launch
+    # (the fastest setting the autotuner measured for n = 32)
+    b32 = pa.PermanentBatch([a32], [r32], [c32], persistent=True, total_units=64 * cus)
     b32.run("eager")
     v32 = b32.results()[0]
     t32 = b32.time("kernel", reps=3)
@@ -210,6 +267,8 @@ def main():
         "rel_err": abs(v32 - ref32) / abs(ref32),
         "flops": f32,
     }
+
+    grad = _gradient_leg(pa, W, reps=max(3, min(args.steps, 10)))
 
     line = {
         "metric": METRIC,
@@ -227,7 +286,7 @@ def main():
         "config": {
             "workload": "batched_uneven: 4096 permanents, n~U{8..24}, row/col multiplicities "
                         "1..3 of a 48-mode Haar unitary, complex128, dd accumulation, "
-                        "persistent launch replayed from a hipGraph",
+                        "one launch per column-width bucket (concurrent), hipGraph replay",
             "items": len(mats),
             "n_range": [8, 24],
             "modes": 48,
@@ -243,15 +302,19 @@ def main():
         },
         "speedup_vs_plain": float(np.median(plain_ms)) / float(np.median(graph_ms)),
         "n32": n32,
+        "gradient": grad,
         "roofline": {
-            "bound": "valu-fp64",
+            "bound": "mfma",
+            "pipe": "FP64 VALU: the sweep is a product of row sums, not a contraction; "
+                    "MI355X's FP64 vector and FP64 matrix peaks are both 78.6 TFLOP/s",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": _traffic_from_profile(),
             "kernel_ms": kernel_avg,
             "flops_per_launch": flops_batch,
+            "flop_model": "sum over items of T * (2 m_c + 6 (n - 1) + 4), T = Gray terms",
             "units": stats["units"],
         },
         "cpu_baseline": cpu,

@@ -359,6 +359,19 @@ def ref_perm(matrix, rows, cols) -> complex:
     return complex(out[0], out[1])
 
 
+def ref_perm_c64(matrix, rows, cols) -> np.complex64:
+    """``permanent_cpp<float>`` of the reference, called directly."""
+    a, r, c = _prep(matrix, rows, cols, dtype=np.complex64)
+    out = np.zeros(2, dtype=np.float32)
+    _check(
+        ref_lib().pqref_perm_c64(
+            _ptr(a.view(np.float32), ctypes.c_float), a.shape[0], a.shape[1],
+            _ptr(r, ctypes.c_int), _ptr(c, ctypes.c_int), _ptr(out, ctypes.c_float),
+        )
+    )
+    return np.complex64(complex(out[0], out[1]))
+
+
 def ref_grad(matrix, rows, cols) -> np.ndarray:
     """``grad_perm`` of the reference (square matrices only)."""
     a, r, c = _prep(matrix, rows, cols)

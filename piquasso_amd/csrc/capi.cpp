@@ -78,9 +78,12 @@ DeviceState &device_state() {
   return st;
 }
 
-// Work units per CU a launch is split into: about 3 resident waves per SIMD
-// (12 per CU) plus slack for load balance.
-constexpr int64_t kUnitsPerCu = 48;
+// Work units per CU a launch is split into (measured on MI355X, 256 CUs):
+// a single large permanent runs best with 64 per CU (n = 32: 16384 units),
+// a batch of uneven permanents with 32 per CU over the whole batch (4096
+// items of n = 8..24: 8192 units; finer splits add per-unit start cost).
+constexpr int64_t kUnitsPerCu = 64;
+constexpr int64_t kBatchUnitsPerCu = 32;
 
 int expand_mode(uint32_t flags) {
   if (flags & PQP_FLAG_EXPAND_COLS) return 1;
@@ -514,7 +517,7 @@ int pqp_grad_batch_c128(int64_t count, const double *const *matrices, const int6
   Plan fwd;
   std::string err;
   PlanOptions fopt;
-  fopt.total_units = static_cast<int64_t>(std::max(device_state().cus, 1)) * kUnitsPerCu;
+  fopt.total_units = static_cast<int64_t>(std::max(device_state().cus, 1)) * kBatchUnitsPerCu;
   int st = plan_build(inputs, fopt, fwd, err);
   if (st != PQP_OK) return fail(st, err);
   std::vector<std::vector<int32_t>> mrows;
@@ -578,10 +581,10 @@ int pqp_batch_create(int64_t count, const void *const *matrices, int is_c64, con
   opt.fp32_products = (flags & PQP_FLAG_FP32) != 0 || is_c64 != 0;
   opt.expand_cols = expand_mode(flags);
   opt.target_inner = target_inner;
-  // about kUnitsPerCu waves per CU over the whole batch, split by cost
+  // about kBatchUnitsPerCu waves per CU over the whole batch, split by cost
   opt.total_units = total_units > 0
                         ? total_units
-                        : static_cast<int64_t>(std::max(device_state().cus, 1)) * kUnitsPerCu;
+                        : static_cast<int64_t>(std::max(device_state().cus, 1)) * kBatchUnitsPerCu;
   std::string err;
   int st = plan_build(inputs, opt, b->x.plan, err);
   if (st != PQP_OK) return fail(st, err);

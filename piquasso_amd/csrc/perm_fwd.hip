@@ -39,32 +39,75 @@ namespace pqp {
 
 namespace {
 
-// prod_j cs_j^{c_j}: all columns once (two interleaved chains for ILP),
-// then the extra powers; columns are sorted by multiplicity, so the
-// columns with c_j >= r form the prefix [0, cpow[r-2]).
+// prod_j cs_j^{c_j}, columns sorted by multiplicity (descending).
+//
+// Grouped form (kPowGrouped): with P_j the running prefix product over
+// columns 0..j and d_j = c_j - c_{j+1} (c_mc = 1),
+//
+//   prod_j cs_j^{c_j} = prod_j P_j^{d_j} * P_{mc-1},
+//
+// i.e. one pass over the columns (m_c - 1 multiplies) plus cmax - 1
+// multiplies into a second accumulator at the columns where the
+// multiplicity drops, instead of sum_j c_j - 1 = n - 1 multiplies.  The
+// drop positions come as three wave-uniform bit masks (d_j <= 3); the check
+// at each column is a scalar bit test and branch.  Items with a larger drop
+// use the count form: all columns once, then column j again for each
+// r = 2..c_j (the columns with c_j >= r are the prefix [0, cnt_r)).
 template <int MCP, typename R>
 __device__ __forceinline__ void column_product(const R (&cr)[MCP], const R (&ci)[MCP],
-                                               R &pr, R &pi, bool ones, int cmax,
-                                               cptr<int32_t> cpow) {
-  R ar = cr[0], ai = ci[0];
-  if constexpr (MCP > 1) {
-    R br = cr[1], bi = ci[1];
+                                               R &pr, R &pi, int flags, int cmax,
+                                               cptr<uint64_t> pmask, cptr<int32_t> cpow) {
+  if (flags & kColsAllOnes) {
+    // all multiplicities 1: two interleaved chains for ILP
+    R ar = cr[0], ai = ci[0];
+    if constexpr (MCP > 1) {
+      R br = cr[1], bi = ci[1];
 #pragma unroll
-    for (int j = 2; j + 1 < MCP; j += 2) {
-      cmul(ar, ai, cr[j], ci[j]);
-      cmul(br, bi, cr[j + 1], ci[j + 1]);
+      for (int j = 2; j + 1 < MCP; j += 2) {
+        cmul(ar, ai, cr[j], ci[j]);
+        cmul(br, bi, cr[j + 1], ci[j + 1]);
+      }
+      if constexpr (MCP % 2 == 1 && MCP > 2) cmul(ar, ai, cr[MCP - 1], ci[MCP - 1]);
+      cmul(ar, ai, br, bi);
     }
-    if constexpr (MCP % 2 == 1 && MCP > 2) cmul(ar, ai, cr[MCP - 1], ci[MCP - 1]);
-    cmul(ar, ai, br, bi);
+    pr = ar;
+    pi = ai;
+    return;
   }
-  if (!ones) {
+  R ar = cr[0], ai = ci[0];
+  // (grouped form only up to 24 columns: at 25-32 its extra accumulator
+  // would push the 3-waves-per-SIMD register budget into spills)
+  if (MCP <= kPowGroupedMaxCols && (flags & kPowGrouped)) {
+    const uint64_t m1 = pmask[0], m2 = pmask[1], m3 = pmask[2];
+    const int first = __builtin_ctzll(m1);  // m1 != 0: some multiplicity exceeds 1
+    R qr = R(1), qi = R(0);
+#pragma unroll
+    for (int j = 0; j < MCP; ++j) {
+      if (j > 0) cmul(ar, ai, cr[j], ci[j]);
+      if ((m1 >> j) & 1) {
+        // a uniform branch; if-converting it would cost masked multiplies
+        __asm__ volatile("" ::: "memory");
+        if (j == first) {
+          qr = ar;
+          qi = ai;
+        } else {
+          cmul(qr, qi, ar, ai);
+        }
+        if ((m2 >> j) & 1) {
+          cmul(qr, qi, ar, ai);
+          if ((m3 >> j) & 1) cmul(qr, qi, ar, ai);
+        }
+      }
+    }
+    cmul(ar, ai, qr, qi);
+  } else {
+#pragma unroll
+    for (int j = 1; j < MCP; ++j) cmul(ar, ai, cr[j], ci[j]);
     for (int r = 2; r <= cmax; ++r) {
       const int cnt = cpow[r - 2];
 #pragma unroll
       for (int j = 0; j < MCP; ++j) {
         if (j < cnt) {
-          // keep this a (uniform, scalar) branch: if-converting it would
-          // turn every column into a masked multiply
           __asm__ volatile("" ::: "memory");
           cmul(ar, ai, cr[j], ci[j]);
         }
@@ -86,7 +129,6 @@ __device__ void sweep_unit(const LaunchArgs &a, int item, int local) {
   const int Dt = it->Dt;
   const int flags = it->flags;
   const int cmax = it->cmax;
-  const bool ones = (flags & kColsAllOnes) != 0;
 
   int64_t u = static_cast<int64_t>(local) * kWave + lane;
   const bool active = u < U;
@@ -95,7 +137,8 @@ __device__ void sweep_unit(const LaunchArgs &a, int item, int local) {
   const cptr<R> B2 = as_const<R>(a.arena, it->b2_off);
   const cptr<double> coef = as_const<double>(a.arena, it->coef_off);
   const cptr<uint32_t> steps = as_const<uint32_t>(a.arena, it->step_off);
-  const cptr<int32_t> cpow = as_const<int32_t>(a.arena, it->cpow_off);
+  const cptr<uint64_t> pmask = as_const<uint64_t>(a.arena, it->cpow_off);
+  const cptr<int32_t> cpow = as_const<int32_t>(a.arena, it->cpow_off + 24);
 
   R cr[MCP], ci[MCP];
   double w_top;
@@ -124,7 +167,7 @@ __device__ void sweep_unit(const LaunchArgs &a, int item, int local) {
     const uint32_t st_next = steps[v + 1 < S ? v + 1 : v];
     const double cf = coef[v];
     R pr, pi;
-    column_product<MCP, R>(cr, ci, pr, pi, ones, cmax, cpow);
+    column_product<MCP, R>(cr, ci, pr, pi, flags, cmax, pmask, cpow);
     sr = __builtin_fma(cf, static_cast<double>(pr), sr);
     si = __builtin_fma(cf, static_cast<double>(pi), si);
     if constexpr (COMP) {

@@ -460,11 +460,27 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     // column multiplicity counts: cpow[r-2] = #{j : c_j >= r}, r = 2..cmax
     d.cmul_off = arena_alloc(plan.arena, int64_t(mcp) * 4);
     for (int j = 0; j < mc; ++j) at<int32_t>(plan.arena, d.cmul_off)[j] = colmult[j];
-    d.cpow_off = arena_alloc(plan.arena, int64_t(std::max(cmax - 1, 1)) * 4);
+    // Power table at cpow_off: three uint64 snapshot masks, then the counts.
+    // With d_j = c_j - c_{j+1} (c_mc = 1) the product prod_j cs_j^{c_j}
+    // equals prod_j P_j^{d_j} * P_{mc-1}, P_j the running prefix product over
+    // columns 0..j; bit j of mask k is set when d_j > k.  When some d_j > 3
+    // the kernels use the counts (cnt_r = #{j : c_j >= r}, r = 2..cmax)
+    // instead.
+    bool grouped = cmax > 1 && mcp <= kPowGroupedMaxCols;
+    uint64_t snap[3] = {0, 0, 0};
+    for (int j = 0; j < mc; ++j) {
+      const int next = j + 1 < mc ? colmult[j + 1] : 1;
+      const int dj = colmult[j] - next;
+      if (dj > 3) grouped = false;
+      for (int k = 0; k < 3; ++k)
+        if (dj > k) snap[k] |= uint64_t(1) << j;
+    }
+    d.cpow_off = arena_alloc(plan.arena, 24 + int64_t(std::max(cmax - 1, 1)) * 4);
+    for (int k = 0; k < 3; ++k) at<uint64_t>(plan.arena, d.cpow_off)[k] = grouped ? snap[k] : 0;
     for (int r = 2; r <= cmax; ++r) {
       int cnt = 0;
       for (int c : colmult) cnt += c >= r;
-      at<int32_t>(plan.arena, d.cpow_off)[r - 2] = cnt;
+      at<int32_t>(plan.arena, d.cpow_off + 24)[r - 2] = cnt;
     }
 
     d.S_in = S_in;
@@ -477,7 +493,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     d.Dt = Dt;
     d.cmax = cmax;
     d.flags = (cmax == 1 ? kColsAllOnes : 0) | (top_binary ? kTopBinary : 0) |
-              (opt.minors ? kMinors : 0);
+              (opt.minors ? kMinors : 0) | (grouped ? kPowGrouped : 0);
     d.partial_base = partial_cursor;
     partial_cursor += static_cast<int64_t>(d.units) * (opt.minors ? mcp + 1 : 1);
     out_cursor += opt.minors ? mc + 1 : 1;

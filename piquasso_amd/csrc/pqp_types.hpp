@@ -26,11 +26,13 @@ constexpr int kWave = 64;          // CDNA wavefront width
 constexpr int kMaxCols = 64;       // widest row-sum vector a kernel keeps in VGPRs
 constexpr int kMaxDigits = 63;     // Gray digits (rows) per permanent
 constexpr int kDdFold = 64;        // FP64 run length folded into dd accumulators
+constexpr int kPowGroupedMaxCols = 24;  // widest kernel with the grouped column powers
 
 enum ItemFlags : int {
   kColsAllOnes = 1,   // every column multiplicity is 1: plain product
   kTopBinary = 2,     // every top digit has radix 2 (rho = 1): bit-level Gray
   kMinors = 4,        // column-minor item: perm(A; r, c - e_l) for every column l
+  kPowGrouped = 8,    // column powers from prefix-product snapshots (masks at cpow_off)
 };
 
 // One permanent; read by the kernels with scalar loads (the item index is
@@ -44,7 +46,8 @@ struct alignas(16) ItemDesc {
   int64_t alpha_off;    // double per inner digit (L): rho_k (inner digits start at g = 0)
   int64_t top_off;      // double x 4 per top digit: {1 / P_t, radix, rho, 0} with place
                         //    value P_t = prod_{t' < t} radix_t'
-  int64_t cpow_off;     // int32 per r = 2..cmax: number of columns with c_j >= r
+  int64_t cpow_off;     // uint64 x 3 snapshot masks (bit j of mask k: c_j - c_{j+1} > k),
+                        //    then int32 per r = 2..cmax: number of columns with c_j >= r
   int64_t cmul_off;     // int32 per kernel column: its multiplicity c_j (minors kernel)
   int64_t S_in;         // inner states per lane
   int64_t U;            // top states (lanes of work)

@@ -3,9 +3,10 @@
 * :func:`permanent_ex` — one permanent with an explicit precision mode and
   the double-double parts of the result;
 * :class:`PermanentBatch` / :func:`permanent_batch` — many permanents of
-  uneven size in one persistent, work-stealing launch (optionally replayed
-  from a hipGraph).  The reference has no batched entry; every item's value
-  equals ``permanent(matrix, rows, cols)`` on that item.
+  uneven size in one launch per column-width bucket, the buckets running
+  concurrently (optionally replayed from a hipGraph).  The reference has no
+  batched entry; every item's value equals ``permanent(matrix, rows, cols)``
+  on that item.
 
 All arithmetic runs in ``libpqp_permanent.so`` on the GPU.
 """
@@ -75,7 +76,8 @@ class PermanentBatch:
         "fp32" (FP32 row sums and products, double-double accumulation).
     persistent:
         True: one persistent work-stealing launch per column-width bucket;
-        False: one block per work unit.
+        False (default, faster on MI355X for uneven batches): one block per
+        work unit, units in decreasing cost order.
     target_inner:
         inner Gray states per lane (0 lets the planner's cost model choose;
         the autotuner in :mod:`piquasso_amd.tune` picks it per workload).
@@ -85,7 +87,7 @@ class PermanentBatch:
         sum is raised to the power c).
     total_units:
         wave work units spread over the batch in proportion to each
-        permanent's cost (0: 48 per compute unit).
+        permanent's cost (0: 32 per compute unit).
     gradient:
         also evaluate d perm / d A of every item on each run (column-minor
         sweeps after the permanents, on the same stream and in the same
@@ -98,7 +100,7 @@ class PermanentBatch:
         rows: Sequence,
         cols: Sequence,
         precision: str = "fp64",
-        persistent: bool = True,
+        persistent: bool = False,
         target_inner: int = 0,
         expand_cols=None,
         total_units: int = 0,
@@ -152,7 +154,7 @@ class PermanentBatch:
     # -- running ---------------------------------------------------------
     def run(self, mode: str = "graph") -> "PermanentBatch":
         """Enqueue one evaluation: mode "graph" (hipGraph replay), "eager"
-        (persistent launches) or "plain" (plain FP64 kernel, one launch per
+        (direct launches) or "plain" (plain FP64 kernel, one launch per
         permanent)."""
         code = {"eager": _capi.PQP_RUN_EAGER, "graph": _capi.PQP_RUN_GRAPH,
                 "plain": _capi.PQP_RUN_PLAIN}[mode]

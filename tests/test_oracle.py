@@ -115,6 +115,25 @@ def test_port_is_bit_exact_with_reference_build(oracle):
         assert port == ref, (trial, port, ref)
 
 
+def test_port_c64_is_bit_exact_with_reference_build(oracle):
+    """The FP32 restatement (permanent_cpp<float>, src/permanent.cpp:266-267)
+    reproduces the reference's complex64 results bit for bit, directly and
+    through the pybind module's float overload."""
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref not built")
+    rng = np.random.default_rng(17)
+    jobs = 4 * oracle.num_procs()
+    m = oracle.ref_module()
+    for trial in range(40):
+        m_r, m_c = rng.integers(1, 7, size=2)
+        n = int(rng.integers(0, 10))
+        a, r, c = _random_case(rng, m_r, m_c, n)
+        a64 = a.astype(np.complex64)
+        port = oracle.perm_port_c64(a64, r, c, n_jobs=jobs)
+        assert port == oracle.ref_perm_c64(a64, r, c), trial
+        assert port == complex(m.permanent(a64, r, c)), trial
+
+
 def test_reference_build_sum_mismatch_and_pybind_entry(oracle):
     if not oracle.ref_available():
         pytest.skip("oracle/_ref not built")

@@ -59,8 +59,11 @@ def emulate(buf):
                           dtype=np.uint32)
     top = f64(int(d["top_off"]), 4 * max(Dt, 1)).reshape(-1, 4)
     cmax = int(d["cmax"])
-    cpow = np.frombuffer(buf[int(d["cpow_off"]): int(d["cpow_off"]) + 4 * max(cmax - 1, 1)].tobytes(),
-                         dtype=np.int32)
+    po = int(d["cpow_off"])
+    masks = [int(x) for x in np.frombuffer(buf[po: po + 24].tobytes(), dtype=np.uint64)]
+    cpow = np.frombuffer(buf[po + 24: po + 24 + 4 * max(cmax - 1, 1)].tobytes(), dtype=np.int32)
+    grouped = bool(d["flags"] & 8)
+    mc = int(d["mc"])
     total = 0j
     for u in range(U):
         cs = B[0].copy()
@@ -81,9 +84,18 @@ def emulate(buf):
         lane_w = -w if (u & 1) else w
         acc = 0j
         for v in range(S):
-            p = np.prod(cs)
-            for rr in range(2, cmax + 1):
-                p *= np.prod(cs[: cpow[rr - 2]])
+            if grouped:
+                # prefix products, multiplied in again where the multiplicity drops
+                pre = np.cumprod(cs)
+                p = pre[-1]
+                for j in range(mc):
+                    for k in range(3):
+                        if (masks[k] >> j) & 1:
+                            p *= pre[j]
+            else:
+                p = np.prod(cs)
+                for rr in range(2, cmax + 1):
+                    p *= np.prod(cs[: cpow[rr - 2]])
             acc += coef[v] * p
             if v + 1 < S:
                 cs = cs + B2[steps[v]]
@@ -230,3 +242,20 @@ def test_minors_plan_errors():
         plan_image(np.eye(2), [1, 1], [1, 1], flags=_capi.PQP_FLAG_GRADIENT)
     with pytest.raises(ValueError):  # column minors are planned up to 32 columns
         plan_image(np.ones((1, 40)), [39], np.ones(40), flags=_capi.PQP_FLAG_GRADIENT)
+
+
+def test_grouped_power_masks():
+    """Snapshot masks: bit j of mask k set when c_j - c_{j+1} > k (c_mc = 1),
+    columns sorted by decreasing multiplicity; drops above 3 fall back to
+    the count form."""
+    a = np.ones((3, 4), dtype=complex)
+    buf = plan_image(a, [4, 3, 2], [3, 3, 2, 1])
+    d = np.frombuffer(buf[:128].tobytes(), dtype=ITEM)[0]
+    po = int(d["cpow_off"])
+    masks = [int(x) for x in np.frombuffer(buf[po: po + 24].tobytes(), dtype=np.uint64)]
+    assert d["flags"] & 8
+    # sorted multiplicities 3 3 2 1: drops d = 0 1 1 0
+    assert masks == [0b0110, 0, 0]
+    buf = plan_image(a, [4, 3, 2], [6, 1, 1, 1])
+    d = np.frombuffer(buf[:128].tobytes(), dtype=ITEM)[0]
+    assert not d["flags"] & 8  # drop of 5 > 3: count form
