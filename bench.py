@@ -212,7 +212,7 @@ def main():
             from piquasso_amd import _capi
 
             cus = _capi.device_info()[1]
-            b = pa.PermanentBatch([a32], [r32], [c32], persistent=True, total_units=64 * cus)
+            b = pa.PermanentBatch([a32], [r32], [c32], persistent=False, total_units=128 * cus)
         ms = b.time("eager", reps=args.steps)
         print(json.dumps({"profile": args.profile, "ms": ms.tolist()}))
         return 0
@@ -252,12 +252,12 @@ def main():
 
     # single large permanent, n = 32 (2^31 Gray terms)
     a32, r32, c32, ref32 = _load_n32()
-    # one large permanent: 64 work units per CU and the work-stealing launch
-    # (the fastest setting the autotuner measured for n = 32)
-    b32 = pa.PermanentBatch([a32], [r32], [c32], persistent=True, total_units=64 * cus)
+    # one large permanent: 128 work units per CU, one block per unit (the
+    # fastest setting measured for n = 32 on MI355X)
+    b32 = pa.PermanentBatch([a32], [r32], [c32], persistent=False, total_units=128 * cus)
     b32.run("eager")
     v32 = b32.results()[0]
-    t32 = b32.time("kernel", reps=3)
+    t32 = b32.time("kernel", reps=5)
     f32 = W.glynn_flops(r32, c32)
     t32_ms = float(np.median(t32))
     n32 = {

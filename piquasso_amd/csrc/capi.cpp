@@ -79,7 +79,7 @@ DeviceState &device_state() {
 }
 
 // Work units per CU a launch is split into (measured on MI355X, 256 CUs):
-// a single large permanent runs best with 64 per CU (n = 32: 16384 units),
+// a single large permanent runs best with 64-128 per CU (n = 32: 16384 units),
 // a batch of uneven permanents with 32 per CU over the whole batch (4096
 // items of n = 8..24: 8192 units; finer splits add per-unit start cost).
 constexpr int64_t kUnitsPerCu = 64;

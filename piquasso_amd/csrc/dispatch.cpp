@@ -55,6 +55,10 @@ hipError_t launch_minors(int bucket, const LaunchArgs &a, int first_unit, int n_
     case 1: return launch_minors_b1(a, first_unit, n_units, stream);
     case 2: return launch_minors_b2(a, first_unit, n_units, stream);
     case 3: return launch_minors_b3(a, first_unit, n_units, stream);
+    case 4: return launch_minors_b4(a, first_unit, n_units, stream);
+    case 5: return launch_minors_b5(a, first_unit, n_units, stream);
+    case 6: return launch_minors_b6(a, first_unit, n_units, stream);
+    case 7: return launch_minors_b7(a, first_unit, n_units, stream);
     default: return hipErrorInvalidValue;
   }
 }

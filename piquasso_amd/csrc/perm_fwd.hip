@@ -49,14 +49,23 @@ namespace {
 // i.e. one pass over the columns (m_c - 1 multiplies) plus cmax - 1
 // multiplies into a second accumulator at the columns where the
 // multiplicity drops, instead of sum_j c_j - 1 = n - 1 multiplies.  The
-// drop positions come as three wave-uniform bit masks (d_j <= 3); the check
-// at each column is a scalar bit test and branch.  Items with a larger drop
-// use the count form: all columns once, then column j again for each
-// r = 2..c_j (the columns with c_j >= r are the prefix [0, cnt_r)).
+// drop positions come as three wave-uniform bit masks (bit j of m[k]:
+// d_j > k; d_j <= 3), loaded once per unit; the test at each column is a
+// scalar bit test and a branch that is rarely taken (the empty volatile asm
+// keeps the compiler from if-converting the rare work into masked
+// multiplies at every column).  Items with a larger drop, or wider than
+// kPowGroupedMaxCols, use the count form: all columns once, then column j
+// again for each r = 2..c_j (columns with c_j >= r are the prefix
+// [0, cnt_r)).
+struct PowState {
+  uint32_t m1, m2, m3;  // grouped form: snapshot masks
+  int first;            // lowest set bit of m1
+};
+
 template <int MCP, typename R>
 __device__ __forceinline__ void column_product(const R (&cr)[MCP], const R (&ci)[MCP],
                                                R &pr, R &pi, int flags, int cmax,
-                                               cptr<uint64_t> pmask, cptr<int32_t> cpow) {
+                                               const PowState &ps, cptr<int32_t> cpow) {
   if (flags & kColsAllOnes) {
     // all multiplicities 1: two interleaved chains for ILP
     R ar = cr[0], ai = ci[0];
@@ -75,27 +84,36 @@ __device__ __forceinline__ void column_product(const R (&cr)[MCP], const R (&ci)
     return;
   }
   R ar = cr[0], ai = ci[0];
-  // (grouped form only up to 24 columns: at 25-32 its extra accumulator
-  // would push the 3-waves-per-SIMD register budget into spills)
   if (MCP <= kPowGroupedMaxCols && (flags & kPowGrouped)) {
-    const uint64_t m1 = pmask[0], m2 = pmask[1], m3 = pmask[2];
-    const int first = __builtin_ctzll(m1);  // m1 != 0: some multiplicity exceeds 1
+    // The masks pass through an opaque copy each term: otherwise the
+    // compiler hoists one precomputed condition per column out of the loop
+    // into SGPR pairs, and the scalar register file overflows.
+    uint32_t m1 = __builtin_amdgcn_readfirstlane(ps.m1);
+    uint32_t m2 = __builtin_amdgcn_readfirstlane(ps.m2);
+    uint32_t m3 = __builtin_amdgcn_readfirstlane(ps.m3);
+    int first = __builtin_amdgcn_readfirstlane(ps.first);
+    __asm__ volatile("" : "+s"(m1), "+s"(m2), "+s"(m3), "+s"(first));
     R qr = R(1), qi = R(0);
 #pragma unroll
     for (int j = 0; j < MCP; ++j) {
       if (j > 0) cmul(ar, ai, cr[j], ci[j]);
-      if ((m1 >> j) & 1) {
-        // a uniform branch; if-converting it would cost masked multiplies
-        __asm__ volatile("" ::: "memory");
+      if (__builtin_expect((m1 >> j) & 1u, 0)) {
+        __asm__ volatile("");
         if (j == first) {
+          __asm__ volatile("");
           qr = ar;
           qi = ai;
         } else {
+          __asm__ volatile("");
           cmul(qr, qi, ar, ai);
         }
-        if ((m2 >> j) & 1) {
+        if (__builtin_expect((m2 >> j) & 1u, 0)) {
+          __asm__ volatile("");
           cmul(qr, qi, ar, ai);
-          if ((m3 >> j) & 1) cmul(qr, qi, ar, ai);
+          if ((m3 >> j) & 1u) {
+            __asm__ volatile("");
+            cmul(qr, qi, ar, ai);
+          }
         }
       }
     }
@@ -108,7 +126,7 @@ __device__ __forceinline__ void column_product(const R (&cr)[MCP], const R (&ci)
 #pragma unroll
       for (int j = 0; j < MCP; ++j) {
         if (j < cnt) {
-          __asm__ volatile("" ::: "memory");
+          __asm__ volatile("");
           cmul(ar, ai, cr[j], ci[j]);
         }
       }
@@ -120,14 +138,14 @@ __device__ __forceinline__ void column_product(const R (&cr)[MCP], const R (&ci)
 
 // One wave unit: top states [64*local, 64*local+64) of item `item`.
 template <int MCP, typename R, bool COMP>
-__device__ void sweep_unit(const LaunchArgs &a, int item, int local) {
+__device__ __forceinline__ void sweep_unit(const LaunchArgs &a, int item, int local) {
   const cptr<ItemDesc> it = as_const<ItemDesc>(a.items) + item;
   const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
   const int64_t U = it->U;
-  const int64_t S = it->S_in;
+  const int S = uniform(static_cast<int>(it->S_in));  // <= 2^16 inner states
   const int L = it->L;
   const int Dt = it->Dt;
-  const int flags = it->flags;
+  const int flags = uniform(it->flags);
   const int cmax = it->cmax;
 
   int64_t u = static_cast<int64_t>(local) * kWave + lane;
@@ -135,10 +153,16 @@ __device__ void sweep_unit(const LaunchArgs &a, int item, int local) {
   if (!active) u = U - 1;
 
   const cptr<R> B2 = as_const<R>(a.arena, it->b2_off);
-  const cptr<double> coef = as_const<double>(a.arena, it->coef_off);
-  const cptr<uint32_t> steps = as_const<uint32_t>(a.arena, it->step_off);
-  const cptr<uint64_t> pmask = as_const<uint64_t>(a.arena, it->cpow_off);
+  const cptr<InnerEntry> tab = as_const<InnerEntry>(a.arena, it->step_off);
   const cptr<int32_t> cpow = as_const<int32_t>(a.arena, it->cpow_off + 24);
+  PowState ps{0u, 0u, 0u, 0};
+  if (flags & kPowGrouped) {
+    const cptr<uint64_t> pm = as_const<uint64_t>(a.arena, it->cpow_off);
+    ps.m1 = static_cast<uint32_t>(pm[0]);
+    ps.m2 = static_cast<uint32_t>(pm[1]);
+    ps.m3 = static_cast<uint32_t>(pm[2]);
+    ps.first = __builtin_ctz(ps.m1 | 0x80000000u);
+  }
 
   R cr[MCP], ci[MCP];
   double w_top;
@@ -147,58 +171,57 @@ __device__ void sweep_unit(const LaunchArgs &a, int item, int local) {
   double lane_w = (u & 1) ? -w_top : w_top;
   if (!active) lane_w = 0.0;
 
-  // Inner sweep.  The row added after step v's product is fetched before
-  // that product (its first PF columns; the scalar registers hold at most
-  // 16 complex values), and the next step's row index one step ahead, so the
-  // scalar-load latency hides behind the product's VALU work.
+  // Inner sweep in blocks of kDdFold states: FP64 run sums inside a block,
+  // folded into the double-double accumulators between blocks.  The row
+  // added after state v (its first PF columns; the scalar registers hold at
+  // most 16 complex values) is fetched before state v's product, and the
+  // table entry of state v + 1 one state ahead, so scalar-load latency
+  // hides behind the product's VALU work.  The update after the last state
+  // uses the padding entry (row 0) and is never read.
   constexpr int PF = MCP < 16 ? MCP : 16;
-  double sr = 0.0, si = 0.0;
   dd accr{0.0, 0.0}, acci{0.0, 0.0};
-  int fold = 0;
-  uint32_t st = steps[0];  // the table has S entries (the last one is padding)
-  for (int64_t v = 0;; ++v) {
-    const cptr<R> row = B2 + static_cast<int64_t>(st) * 2 * MCP;
-    R ur[PF], ui[PF];
+  InnerEntry e = inner_entry(tab, 0);
+  for (int v0 = 0; v0 < S; v0 += kDdFold) {
+    const int vend = v0 + kDdFold < S ? v0 + kDdFold : S;
+    double sr = 0.0, si = 0.0;
+    for (int v = v0; v < vend; ++v) {
+      const cptr<R> row = B2 + static_cast<int64_t>(e.row) * (2 * MCP);
+      R ur[PF], ui[PF];
 #pragma unroll
-    for (int j = 0; j < PF; ++j) {
-      ur[j] = row[2 * j];
-      ui[j] = row[2 * j + 1];
-    }
-    const uint32_t st_next = steps[v + 1 < S ? v + 1 : v];
-    const double cf = coef[v];
-    R pr, pi;
-    column_product<MCP, R>(cr, ci, pr, pi, flags, cmax, pmask, cpow);
-    sr = __builtin_fma(cf, static_cast<double>(pr), sr);
-    si = __builtin_fma(cf, static_cast<double>(pi), si);
-    if constexpr (COMP) {
-      if (++fold == kDdFold) {
-        accr = dd_add_d(accr, sr);
-        acci = dd_add_d(acci, si);
-        sr = 0.0;
-        si = 0.0;
-        fold = 0;
+      for (int j = 0; j < PF; ++j) {
+        ur[j] = row[2 * j];
+        ui[j] = row[2 * j + 1];
       }
-    }
-    if (v + 1 >= S) break;
+      const InnerEntry en = inner_entry(tab, v + 1);
+      R pr, pi;
+      column_product<MCP, R>(cr, ci, pr, pi, flags, cmax, ps, cpow);
+      sr = __builtin_fma(e.coef, static_cast<double>(pr), sr);
+      si = __builtin_fma(e.coef, static_cast<double>(pi), si);
 #pragma unroll
-    for (int j = 0; j < PF; ++j) {
-      cr[j] += ur[j];
-      ci[j] += ui[j];
-    }
+      for (int j = 0; j < PF; ++j) {
+        cr[j] += ur[j];
+        ci[j] += ui[j];
+      }
 #pragma unroll
-    for (int j = PF; j < MCP; ++j) {
-      cr[j] += row[2 * j];
-      ci[j] += row[2 * j + 1];
+      for (int j = PF; j < MCP; ++j) {
+        cr[j] += row[2 * j];
+        ci[j] += row[2 * j + 1];
+      }
+      e = en;
     }
-    st = st_next;
+    if constexpr (COMP) {
+      accr = dd_add_d(accr, sr);
+      acci = dd_add_d(acci, si);
+    } else {
+      accr.hi += sr;
+      acci.hi += si;
+    }
   }
   cdd lane_sum;
   if constexpr (COMP) {
-    accr = dd_add_d(accr, sr);
-    acci = dd_add_d(acci, si);
     lane_sum = cdd{dd_mul_d(accr, lane_w), dd_mul_d(acci, lane_w)};
   } else {
-    lane_sum = cdd{{sr * lane_w, 0.0}, {si * lane_w, 0.0}};
+    lane_sum = cdd{{accr.hi * lane_w, 0.0}, {acci.hi * lane_w, 0.0}};
   }
   if constexpr (COMP) {
     lane_sum = wave_reduce(lane_sum);

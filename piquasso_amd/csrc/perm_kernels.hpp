@@ -46,6 +46,10 @@ hipError_t launch_minors_b0(const LaunchArgs &a, int first_unit, int n_units, hi
 hipError_t launch_minors_b1(const LaunchArgs &a, int first_unit, int n_units, hipStream_t s);
 hipError_t launch_minors_b2(const LaunchArgs &a, int first_unit, int n_units, hipStream_t s);
 hipError_t launch_minors_b3(const LaunchArgs &a, int first_unit, int n_units, hipStream_t s);
+hipError_t launch_minors_b4(const LaunchArgs &a, int first_unit, int n_units, hipStream_t s);
+hipError_t launch_minors_b5(const LaunchArgs &a, int first_unit, int n_units, hipStream_t s);
+hipError_t launch_minors_b6(const LaunchArgs &a, int first_unit, int n_units, hipStream_t s);
+hipError_t launch_minors_b7(const LaunchArgs &a, int first_unit, int n_units, hipStream_t s);
 hipError_t launch_minors(int bucket, const LaunchArgs &a, int first_unit, int n_units,
                          hipStream_t stream);
 hipError_t launch_minors_finalize(const LaunchArgs &a, int n_items, hipStream_t stream);

@@ -37,12 +37,12 @@ namespace pqp {
 namespace {
 
 template <int MCP, bool ONES>
-__device__ void minors_unit(const LaunchArgs &a, int item, int local) {
+__device__ __forceinline__ void minors_unit(const LaunchArgs &a, int item, int local) {
   using R = double;
   const cptr<ItemDesc> it = as_const<ItemDesc>(a.items) + item;
   const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
   const int64_t U = it->U;
-  const int64_t S = it->S_in;
+  const int S = uniform(static_cast<int>(it->S_in));
   const int L = it->L;
   const int Dt = it->Dt;
   const int flags = it->flags;
@@ -53,8 +53,7 @@ __device__ void minors_unit(const LaunchArgs &a, int item, int local) {
   if (!active) u = U - 1;
 
   const cptr<R> B2 = as_const<R>(a.arena, it->b2_off);
-  const cptr<double> coef = as_const<double>(a.arena, it->coef_off);
-  const cptr<uint32_t> steps = as_const<uint32_t>(a.arena, it->step_off);
+  const cptr<InnerEntry> tab = as_const<InnerEntry>(a.arena, it->step_off);
   const cptr<int32_t> cmult = as_const<int32_t>(a.arena, it->cmul_off);
 
   R cr[MCP], ci[MCP];
@@ -71,88 +70,145 @@ __device__ void minors_unit(const LaunchArgs &a, int item, int local) {
   }
 
   constexpr int PF = MCP < 8 ? MCP : 8;  // row columns fetched ahead of the products
-  uint32_t st = steps[0];
-  for (int64_t v = 0;; ++v) {
-    const cptr<R> row = B2 + static_cast<int64_t>(st) * 2 * MCP;
+  InnerEntry e = inner_entry(tab, 0);
+  for (int v = 0; v < S; ++v) {
+    const cptr<R> row = B2 + static_cast<int64_t>(e.row) * (2 * MCP);
     R ur[PF], ui[PF];
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
       ur[j] = row[2 * j];
       ui[j] = row[2 * j + 1];
     }
-    const uint32_t st_next = steps[v + 1 < S ? v + 1 : v];
-    const double cf = coef[v];
+    const InnerEntry en = inner_entry(tab, v + 1);
+    const double cf = e.coef;
 
-    // F_k = cs_k^{c_k - 1} (1 for unit multiplicity), E_k = F_k cs_k, both
-    // recomputed where needed (wave-uniform powers; no per-column arrays).
-    // Prefix P_k = prod_{k' < k} E_k', suffix Q_k = prod_{k' > k} E_k';
-    // minor product l: P_l F_l Q_l.
-    auto F = [&](int k, R &fr_, R &fi_) {
-      fr_ = cr[k];
-      fi_ = ci[k];
-      const int ck = cmult[k];
-      for (int q = 2; q < ck; ++q) cmul(fr_, fi_, cr[k], ci[k]);
-    };
-    auto E = [&](int k, R &er, R &ei) {
-      er = cr[k];
-      ei = ci[k];
-      if constexpr (!ONES) {
+    if constexpr (ONES) {
+      // unit multiplicities: E_k = cs_k, F_k = 1.  Prefix products
+      // P_k = prod_{k' < k} cs_k' are kept for even k only (P_0 = 1); an odd
+      // P_{2m+1} = P_{2m} cs_{2m} is rebuilt in the backward pass, which
+      // carries the running suffix Q: minor l = P_l Q_l, and the final Q is
+      // the uncut product.  (Half the prefix registers for m_c / 2 extra
+      // multiplies keeps the kernel within one register budget step.)
+      constexpr int NP = (MCP + 1) / 2;  // P_0, P_2, ..., P_{2(NP-1)}
+      R pr[NP], pi[NP];
+      pr[0] = R(1);
+      pi[0] = R(0);
+      if constexpr (NP > 1) {
+        pr[1] = cr[0];
+        pi[1] = ci[0];
+        cmul(pr[1], pi[1], cr[1], ci[1]);
+#pragma unroll
+        for (int m = 2; m < NP; ++m) {
+          pr[m] = pr[m - 1];
+          pi[m] = pi[m - 1];
+          cmul(pr[m], pi[m], cr[2 * m - 2], ci[2 * m - 2]);
+          cmul(pr[m], pi[m], cr[2 * m - 1], ci[2 * m - 1]);
+        }
+      }
+      R qr = R(1), qi = R(0);
+#pragma unroll
+      for (int k = MCP - 1; k >= 0; --k) {
+        R mr, mi;
+        if (k % 2 == 0) {
+          mr = pr[k / 2];
+          mi = pi[k / 2];
+        } else {
+          mr = pr[k / 2];
+          mi = pi[k / 2];
+          if (k > 1) {
+            cmul(mr, mi, cr[k - 1], ci[k - 1]);
+          } else {
+            mr = cr[0];
+            mi = ci[0];
+          }
+        }
+        if (k < MCP - 1) {
+          if (k == 0) {
+            mr = qr;
+            mi = qi;
+          } else {
+            cmul(mr, mi, qr, qi);
+          }
+        }
+        acr[k] = __builtin_fma(cf, mr, acr[k]);
+        aci[k] = __builtin_fma(cf, mi, aci[k]);
+        if (k == MCP - 1) {
+          qr = cr[k];
+          qi = ci[k];
+        } else {
+          cmul(qr, qi, cr[k], ci[k]);
+        }
+      }
+      acr[MCP] = __builtin_fma(cf, qr, acr[MCP]);
+      aci[MCP] = __builtin_fma(cf, qi, aci[MCP]);
+    } else {
+      // F_k = cs_k^{c_k - 1} (1 for unit multiplicity), E_k = F_k cs_k, both
+      // recomputed where needed (wave-uniform powers; no per-column arrays).
+      // Prefix P_k = prod_{k' < k} E_k', suffix Q_k = prod_{k' > k} E_k';
+      // minor product l: P_l F_l Q_l.
+      auto F = [&](int k, R &fr_, R &fi_) {
+        fr_ = cr[k];
+        fi_ = ci[k];
+        const int ck = cmult[k];
+        for (int q = 2; q < ck; ++q) cmul(fr_, fi_, cr[k], ci[k]);
+      };
+      auto E = [&](int k, R &er, R &ei) {
+        er = cr[k];
+        ei = ci[k];
         const int ck = cmult[k];
         for (int q = 1; q < ck; ++q) cmul(er, ei, cr[k], ci[k]);
-      }
-    };
-    R pr[MCP], pi[MCP];
-    pr[0] = R(1);
-    pi[0] = R(0);
-    if constexpr (MCP > 1) {
-      E(0, pr[1], pi[1]);
+      };
+      R pr[MCP], pi[MCP];
+      pr[0] = R(1);
+      pi[0] = R(0);
+      if constexpr (MCP > 1) {
+        E(0, pr[1], pi[1]);
 #pragma unroll
-      for (int k = 2; k < MCP; ++k) {
-        R er, ei;
-        E(k - 1, er, ei);
-        pr[k] = pr[k - 1];
-        pi[k] = pi[k - 1];
-        cmul(pr[k], pi[k], er, ei);
+        for (int k = 2; k < MCP; ++k) {
+          R er, ei;
+          E(k - 1, er, ei);
+          pr[k] = pr[k - 1];
+          pi[k] = pi[k - 1];
+          cmul(pr[k], pi[k], er, ei);
+        }
       }
-    }
-    R qr = R(1), qi = R(0);
+      R qr = R(1), qi = R(0);
 #pragma unroll
-    for (int k = MCP - 1; k >= 0; --k) {
-      R mr, mi;
-      if (k == MCP - 1) {
-        mr = pr[k];
-        mi = pi[k];
-      } else if (k == 0) {
-        mr = qr;
-        mi = qi;
-      } else {
-        mr = pr[k];
-        mi = pi[k];
-        cmul(mr, mi, qr, qi);
-      }
-      if constexpr (!ONES) {
+      for (int k = MCP - 1; k >= 0; --k) {
+        R mr, mi;
+        if (k == MCP - 1) {
+          mr = pr[k];
+          mi = pi[k];
+        } else if (k == 0) {
+          mr = qr;
+          mi = qi;
+        } else {
+          mr = pr[k];
+          mi = pi[k];
+          cmul(mr, mi, qr, qi);
+        }
         if (cmult[k] > 1) {
           R fr_, fi_;
           F(k, fr_, fi_);
           cmul(mr, mi, fr_, fi_);
         }
+        acr[k] = __builtin_fma(cf, mr, acr[k]);
+        aci[k] = __builtin_fma(cf, mi, aci[k]);
+        R er, ei;
+        E(k, er, ei);
+        if (k == MCP - 1) {
+          qr = er;
+          qi = ei;
+        } else {
+          cmul(qr, qi, er, ei);
+        }
       }
-      acr[k] = __builtin_fma(cf, mr, acr[k]);
-      aci[k] = __builtin_fma(cf, mi, aci[k]);
-      R er, ei;
-      E(k, er, ei);
-      if (k == MCP - 1) {
-        qr = er;
-        qi = ei;
-      } else {
-        cmul(qr, qi, er, ei);
-      }
+      // uncut product prod_k E_k = Q_{-1}
+      acr[MCP] = __builtin_fma(cf, qr, acr[MCP]);
+      aci[MCP] = __builtin_fma(cf, qi, aci[MCP]);
     }
-    // uncut product prod_k E_k = Q_{-1}
-    acr[MCP] = __builtin_fma(cf, qr, acr[MCP]);
-    aci[MCP] = __builtin_fma(cf, qi, aci[MCP]);
 
-    if (v + 1 >= S) break;
+    // (after the last state this adds the padding row 0; never read)
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
       cr[j] += ur[j];
@@ -163,7 +219,7 @@ __device__ void minors_unit(const LaunchArgs &a, int item, int local) {
       cr[j] += row[2 * j];
       ci[j] += row[2 * j + 1];
     }
-    st = st_next;
+    e = en;
   }
 
   double *p = a.partials + 4 * (it->partial_base + static_cast<int64_t>(local) * (MCP + 1));
@@ -184,35 +240,41 @@ __device__ void minors_unit(const LaunchArgs &a, int item, int local) {
   }
 }
 
-template <int LO, int HI>
-__device__ __forceinline__ void minors_dispatch(const LaunchArgs &a, int mcp, bool ones,
-                                                int item, int local) {
+template <int LO, int HI, bool ONES>
+__device__ __forceinline__ void minors_dispatch(const LaunchArgs &a, int mcp, int item,
+                                                int local) {
   if constexpr (LO <= HI) {
     if (mcp == LO) {
-      if (ones)
-        minors_unit<LO, true>(a, item, local);
-      else
-        minors_unit<LO, false>(a, item, local);
+      minors_unit<LO, ONES>(a, item, local);
     } else {
-      minors_dispatch<LO + 1, HI>(a, mcp, ones, item, local);
+      minors_dispatch<LO + 1, HI, ONES>(a, mcp, item, local);
     }
   }
 }
 
+// Waves per SIMD the register budget is sized for: unit-multiplicity
+// kernels keep m_c row sums, m_c - 1 prefix products and m_c + 1
+// accumulators (about 12 m_c VGPRs); the general kernels also recompute
+// column powers.
+constexpr int minor_waves(int hi, bool ones) {
+  return hi <= 8 ? 4 : (ones && hi <= 16) ? 2 : 1;
+}
+
 }  // namespace
 
-// One block per work unit.
-template <int LO, int HI>
+// One block per work unit.  ONES: every column multiplicity is 1 (the
+// planner puts such items in the buckets 0-3, the others in 4-7), so the
+// unit-multiplicity kernel's registers are not sized by the general path.
+template <int LO, int HI, bool ONES>
 __global__ __launch_bounds__(kWave)
-__attribute__((amdgpu_waves_per_eu(HI <= 8 ? 3 : 1, 8))) void perm_minors_kernel(
+__attribute__((amdgpu_waves_per_eu(minor_waves(HI, ONES), 8))) void perm_minors_kernel(
     LaunchArgs a, int first_unit) {
   const cptr<WorkUnit> wu = as_const<WorkUnit>(a.units) + (first_unit + static_cast<int>(blockIdx.x));
   const int item = uniform(wu->item);
   const int local = uniform(wu->local);
   const cptr<ItemDesc> it = as_const<ItemDesc>(a.items) + item;
   const int mcp = uniform(it->mcp);
-  const bool ones = (uniform(it->flags) & kColsAllOnes) != 0;
-  minors_dispatch<LO, HI>(a, mcp, ones, item, local);
+  minors_dispatch<LO, HI, ONES>(a, mcp, item, local);
 }
 
 #if PQP_BUCKET_ID == 0
@@ -256,24 +318,26 @@ __global__ __launch_bounds__(kWave) void perm_minors_finalize_kernel(LaunchArgs 
 #endif
 
 // ---------------------------------------------------------------------------
-// Instantiation per column-width bucket (-DPQP_BUCKET_ID=0..3, widths up to 32).
+// Instantiation per bucket (-DPQP_BUCKET_ID=0..7): widths 1-8, 9-16, 17-24,
+// 25-32; buckets 0-3 unit column multiplicities, 4-7 general.
 // ---------------------------------------------------------------------------
 #ifndef PQP_BUCKET_ID
-#error "compile with -DPQP_BUCKET_ID=<0..3>"
+#error "compile with -DPQP_BUCKET_ID=<0..7>"
 #endif
-#if PQP_BUCKET_ID == 0
+#if PQP_BUCKET_ID % 4 == 0
 #define PQP_LO 1
 #define PQP_HI 8
-#elif PQP_BUCKET_ID == 1
+#elif PQP_BUCKET_ID % 4 == 1
 #define PQP_LO 9
 #define PQP_HI 16
-#elif PQP_BUCKET_ID == 2
+#elif PQP_BUCKET_ID % 4 == 2
 #define PQP_LO 17
 #define PQP_HI 24
-#elif PQP_BUCKET_ID == 3
+#else
 #define PQP_LO 25
 #define PQP_HI 32
 #endif
+#define PQP_ONES (PQP_BUCKET_ID < 4)
 
 #define PQP_PASTE_(base, k) base##k
 #define PQP_PASTE(base, k) PQP_PASTE_(base, k)
@@ -281,8 +345,8 @@ __global__ __launch_bounds__(kWave) void perm_minors_finalize_kernel(LaunchArgs 
 hipError_t PQP_PASTE(launch_minors_b, PQP_BUCKET_ID)(const LaunchArgs &a, int first_unit,
                                                       int n_units, hipStream_t stream) {
   if (n_units <= 0) return hipSuccess;
-  hipLaunchKernelGGL((perm_minors_kernel<PQP_LO, PQP_HI>), dim3(n_units), dim3(kWave), 0, stream,
-                     a, first_unit);
+  hipLaunchKernelGGL((perm_minors_kernel<PQP_LO, PQP_HI, PQP_ONES>), dim3(n_units), dim3(kWave), 0,
+                     stream, a, first_unit);
   return hipGetLastError();
 }
 

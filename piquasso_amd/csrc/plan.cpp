@@ -419,23 +419,32 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
       for (int r : profile) radix.push_back(r + 1);
       std::vector<int> sd, sv;
       gray_steps(radix, sd, sv);
-      // S entries: the kernel reads one entry ahead, the last one is padding
-      tb.step_off = arena_alloc(plan.arena, (static_cast<int64_t>(sd.size()) + 1) * 4);
-      tb.coef_off = arena_alloc(plan.arena, S_in * 8);
+      // Interleaved inner table, one 16-byte entry per inner state v:
+      // {double coefficient, uint32 row of the +-2B table added after state v,
+      // uint32 0}.  S_in + 1 entries: the kernels read one entry ahead, and
+      // the entries past the last step are padding (row 0, coefficient 0).
+      tb.step_off = arena_alloc(plan.arena, (S_in + 1) * 16);
+      tb.coef_off = tb.step_off;
+      char *tab = plan.arena.data() + tb.step_off;
+      auto set_entry = [&](int64_t v, double coef, uint32_t row) {
+        std::memcpy(tab + 16 * v, &coef, 8);
+        std::memcpy(tab + 16 * v + 8, &row, 4);
+      };
       std::vector<int> g(L, 0);
       int64_t w = 1;
-      at<double>(plan.arena, tb.coef_off)[0] = 1.0;
+      set_entry(0, 1.0, 0);
       for (size_t s = 0; s < sd.size(); ++s) {
         const int t = sd[s];
         const int prev = g[t];
         g[t] = sv[s];
         // moving g up subtracts 2 b_t from the row sums (row 2t+1 = -2B),
         // moving it down adds 2 b_t (row 2t)
-        at<uint32_t>(plan.arena, tb.step_off)[s] =
-            static_cast<uint32_t>(2 * t + (sv[s] > prev ? 1 : 0));
+        const uint32_t row = static_cast<uint32_t>(2 * t + (sv[s] > prev ? 1 : 0));
+        std::memcpy(tab + 16 * s + 8, &row, 4);
         w = w / binom64(profile[t], prev) * binom64(profile[t], g[t]);
-        double sgn = ((s + 1) & 1) ? -1.0 : 1.0;
-        at<double>(plan.arena, tb.coef_off)[s + 1] = sgn * static_cast<double>(w);
+        const double sgn = ((s + 1) & 1) ? -1.0 : 1.0;
+        const double coef = sgn * static_cast<double>(w);
+        std::memcpy(tab + 16 * (s + 1), &coef, 8);
       }
       it = inner_cache.emplace(profile, tb).first;
     }
@@ -514,7 +523,9 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     plan.total_flops += inf.flops;
     plan.total_terms += T;
     item_units[k] = d.units;
-    item_bucket[k] = bucket;
+    // column-minor items: unit-multiplicity kernels in buckets 0-3, the
+    // general ones in 4-7
+    item_bucket[k] = opt.minors && cmax > 1 ? bucket + 4 : bucket;
   }
   plan.n_partials = partial_cursor;
   plan.n_out = out_cursor;

@@ -41,8 +41,9 @@ struct alignas(16) ItemDesc {
   int64_t b_off;        // B: (1 + L + Dt) rows x mcp complex, row 0 = prepended row,
                         //    rows 1..L inner digits, rows L+1.. top digits (unscaled)
   int64_t b2_off;       // inner +-2B: 2L rows x mcp complex (row 2k = +2B_k, 2k+1 = -2B_k)
-  int64_t step_off;     // uint32 per inner step (S_in - 1): row of the b2 table to add
-  int64_t coef_off;     // double per inner state (S_in): (-1)^v prod_inner C(rho, g)
+  int64_t step_off;     // inner table, S_in + 1 entries of InnerEntry (shared by items
+                        //    with the same inner multiplicity profile)
+  int64_t coef_off;     // the same table (kept for the layout)
   int64_t alpha_off;    // double per inner digit (L): rho_k (inner digits start at g = 0)
   int64_t top_off;      // double x 4 per top digit: {1 / P_t, radix, rho, 0} with place
                         //    value P_t = prod_{t' < t} radix_t'
@@ -64,6 +65,16 @@ struct alignas(16) ItemDesc {
   int32_t pad0;
 };
 static_assert(sizeof(ItemDesc) == 128, "ItemDesc layout");
+
+// Inner Gray state v: its coefficient (-1)^v prod_inner C(rho, g) and the
+// row of the +-2B table added after it (entries past the last step are
+// padding: row 0, coefficient 0).
+struct alignas(16) InnerEntry {
+  double coef;
+  uint32_t row;
+  uint32_t pad;
+};
+static_assert(sizeof(InnerEntry) == 16, "InnerEntry layout");
 
 // A unit of work: one wavefront sweeps top states [64*local, 64*local + 64)
 // of permanent `item`.

@@ -74,6 +74,15 @@ __constant__ double kInvSmall[65] = {
     1.0 / 64};
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Entry v of an inner table (scalar loads).
+__device__ __forceinline__ InnerEntry inner_entry(cptr<InnerEntry> tab, int v) {
+  InnerEntry e;
+  e.coef = tab[v].coef;
+  e.row = tab[v].row;
+  e.pad = 0u;
+  return e;
+}
+
 // Butterfly reduction of a complex dd value over the 64 lanes.
 __device__ __forceinline__ cdd wave_reduce(cdd v) {
 #pragma unroll

@@ -94,3 +94,23 @@ def test_empty_selection_needs_no_device():
 
     assert pa.permanent_laplace(np.ones((2, 2)), [0, 0], [0, 1]).tolist() == [1]
     assert pa.permanent_laplace(np.zeros((0, 3)), [], [1, 0, 0]).tolist() == [1]
+
+
+def test_no_kernel_spills_to_scratch():
+    """Every sweep kernel keeps its state in registers: the build records the
+    compiler's resource usage per kernel object (piquasso_amd/csrc/hipcc_kernel.sh);
+    a scratch spill in the hot loop costs several times the kernel's speed."""
+    import glob
+
+    res = sorted(glob.glob(os.path.join(ROOT, "piquasso_amd", "lib", "obj", "*.res")))
+    if not res:
+        pytest.skip("no resource records (library not built by piquasso_amd/csrc/Makefile)")
+    spills = []
+    for path in res:
+        name = None
+        for line in open(path):
+            if line.startswith("Function Name:"):
+                name = line.split(":", 1)[1].strip()
+            elif line.startswith("ScratchSize") and int(line.split(":")[-1]) != 0:
+                spills.append((os.path.basename(path), name))
+    assert not spills, spills

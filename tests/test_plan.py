@@ -41,6 +41,18 @@ def plan_image(a, rows, cols, target_inner=0, min_units=0, flags=0):
     return buf
 
 
+INNER = np.dtype([("coef", "<f8"), ("row", "<u4"), ("pad", "<u4")])
+
+
+def inner_table(buf, d, S):
+    """The interleaved inner table: per inner state v its coefficient and the
+    +-2B row added after it (S + 1 entries, padding past the last step)."""
+    off = int(d["step_off"])
+    t = np.frombuffer(buf[off: off + 16 * (S + 1)].tobytes(), dtype=INNER)
+    assert t["row"][S - 1] == 0 and t["row"][S] == 0 and t["coef"][S] == 0.0
+    return t["coef"][:S].copy(), t["row"][: max(S - 1, 0)].copy()
+
+
 def emulate(buf):
     """The kernels' arithmetic on the planned arena, in numpy (complex128)."""
     d = np.frombuffer(buf[:128].tobytes(), dtype=ITEM)[0]
@@ -54,9 +66,7 @@ def emulate(buf):
     B = f64(int(d["b_off"]), (1 + L + Dt) * mcp * 2).view(np.complex128).reshape(1 + L + Dt, mcp)
     B2 = f64(int(d["b2_off"]), 2 * max(L, 1) * mcp * 2).view(np.complex128).reshape(-1, mcp)
     alpha = f64(int(d["alpha_off"]), max(L, 1))
-    coef = f64(int(d["coef_off"]), S)
-    steps = np.frombuffer(buf[int(d["step_off"]): int(d["step_off"]) + 4 * max(S - 1, 0)].tobytes(),
-                          dtype=np.uint32)
+    coef, steps = inner_table(buf, d, S)
     top = f64(int(d["top_off"]), 4 * max(Dt, 1)).reshape(-1, 4)
     cmax = int(d["cmax"])
     po = int(d["cpow_off"])
@@ -178,9 +188,7 @@ def emulate_minors(buf, c):
     B = f64(int(d["b_off"]), (1 + L + Dt) * mcp * 2).view(np.complex128).reshape(1 + L + Dt, mcp)
     B2 = f64(int(d["b2_off"]), 2 * max(L, 1) * mcp * 2).view(np.complex128).reshape(-1, mcp)
     alpha = f64(int(d["alpha_off"]), max(L, 1))
-    coef = f64(int(d["coef_off"]), S)
-    steps = np.frombuffer(buf[int(d["step_off"]): int(d["step_off"]) + 4 * max(S - 1, 0)].tobytes(),
-                          dtype=np.uint32)
+    coef, steps = inner_table(buf, d, S)
     top = f64(int(d["top_off"]), 4 * max(Dt, 1)).reshape(-1, 4)
     cm = np.frombuffer(buf[int(d["cmul_off"]): int(d["cmul_off"]) + 4 * mc].tobytes(), dtype=np.int32)
     acc = np.zeros(mc + 1, dtype=np.complex128)
