@@ -132,9 +132,11 @@ int pqp_batch_results(pqp_batch *b, double *out, double *out_dd);
 int pqp_batch_time(pqp_batch *b, int mode, int reps, float *ms);
 
 /* Planning statistics: algorithmic FLOP (sum of T * (2 m_c + 6 (n-1) + 4)),
- * Gray terms, wave work units and device arena size.  Any pointer may be null. */
+ * Gray terms, wave work units, device arena size, and lane terms (sum over
+ * units of 64 lanes x inner states: terms / lane_terms is the fraction of
+ * lane-steps doing useful work).  Any pointer may be null. */
 int pqp_batch_stats(const pqp_batch *b, double *flops, int64_t *terms, int64_t *units,
-                    int64_t *arena_bytes);
+                    int64_t *arena_bytes, int64_t *lane_terms);
 
 /* Gradient batches (PQP_FLAG_GRADIENT): grads[k] receives the m_r x m_c
  * complex matrix d perm_k / d A_k (row-major, interleaved) of the last run. */

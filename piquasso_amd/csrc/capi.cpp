@@ -707,8 +707,14 @@ int pqp_batch_time(pqp_batch *b, int mode, int reps, float *ms) {
 }
 
 int pqp_batch_stats(const pqp_batch *b, double *flops, int64_t *terms, int64_t *units,
-                    int64_t *arena_bytes) {
+                    int64_t *arena_bytes, int64_t *lane_terms) {
   if (!b) return fail(PQP_ERR_INVALID, "batch is null");
+  if (lane_terms) {
+    int64_t lt = 0;
+    for (const Plan *p : {&b->x.plan, &b->xm.plan})
+      for (const ItemDesc &d : p->items) lt += static_cast<int64_t>(d.units) * kWave * d.S_in;
+    *lane_terms = lt;
+  }
   if (flops) *flops = b->x.plan.total_flops + (b->grad ? b->xm.plan.total_flops : 0.0);
   if (terms) *terms = b->x.plan.total_terms + (b->grad ? b->xm.plan.total_terms : 0);
   if (units)

@@ -201,14 +201,16 @@ class PermanentBatch:
         terms = ctypes.c_int64(0)
         units = ctypes.c_int64(0)
         arena = ctypes.c_int64(0)
+        lane_terms = ctypes.c_int64(0)
         _capi.check(
             _capi.lib().pqp_batch_stats(
                 self._h, ctypes.byref(flops), ctypes.byref(terms), ctypes.byref(units),
-                ctypes.byref(arena),
+                ctypes.byref(arena), ctypes.byref(lane_terms),
             )
         )
         return {"flops": flops.value, "terms": terms.value, "units": units.value,
-                "arena_bytes": arena.value}
+                "arena_bytes": arena.value, "lane_terms": lane_terms.value,
+                "lane_efficiency": terms.value / lane_terms.value if lane_terms.value else 1.0}
 
     def close(self) -> None:
         if getattr(self, "_h", None):
