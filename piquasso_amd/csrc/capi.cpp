@@ -78,11 +78,12 @@ DeviceState &device_state() {
   return st;
 }
 
-// Work units per CU a launch is split into (measured on MI355X, 256 CUs):
-// a single large permanent runs best with 64-128 per CU (n = 32: 16384 units),
+// Work units a launch is split into (measured on MI355X, 256 CUs): one
+// permanent of T Gray terms runs best with about 1.4 sqrt(T) units (n = 20:
+// 1024, n = 24: 4096, n = 28: 16384), capped at 128 per CU (n = 32: 32768);
 // a batch of uneven permanents with 32 per CU over the whole batch (4096
 // items of n = 8..24: 8192 units; finer splits add per-unit start cost).
-constexpr int64_t kUnitsPerCu = 64;
+constexpr int64_t kSingleUnitsPerCu = 128;
 constexpr int64_t kBatchUnitsPerCu = 32;
 
 int expand_mode(uint32_t flags) {
@@ -133,18 +134,48 @@ struct DevBuf {
   }
 };
 
+// Pinned host staging buffer (hipHostMalloc): uploads and result downloads
+// go through it as true asynchronous DMA instead of pageable-memory copies.
+struct HostBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t n) {
+    if (n <= bytes) return PQP_OK;
+    if (p) {
+      (void)hipHostFree(p);
+      p = nullptr;
+      bytes = 0;
+    }
+    size_t alloc = std::max<size_t>(n, 65536);
+    hipError_t e = hipHostMalloc(&p, alloc, hipHostMallocDefault);
+    if (e != hipSuccess)
+      return fail(PQP_ERR_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    bytes = alloc;
+    return PQP_OK;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
 // Device image of one plan plus the stream it runs on.
 struct Exec {
   Plan plan;
   std::vector<int32_t> item_first_unit;  // first unit of each item in plan.units
-  DevBuf arena, partials, out, out_dd, heads;
+  DevBuf arena, partials, heads;
+  DevBuf results;   // 2 doubles per result, then 4 (dd parts) per result
+  HostBuf stage;    // pinned staging for the arena upload and result downloads
+  size_t n_results = 0;
+  bool heads_clean = false;  // queue heads zeroed since (re)allocation
   hipStream_t stream = nullptr;
   bool own_stream = false;
   uint32_t flags = 0;
   // side streams and events for running several width buckets concurrently
-  hipStream_t side[kBuckets] = {};
+  hipStream_t side[kMaxRanges] = {};
   hipEvent_t ev_fork = nullptr;
-  hipEvent_t ev_join[kBuckets] = {};
+  hipEvent_t ev_join[kMaxRanges] = {};
 
   LaunchArgs args() const {
     LaunchArgs a;
@@ -152,8 +183,8 @@ struct Exec {
     a.items = reinterpret_cast<const ItemDesc *>(static_cast<const char *>(arena.p) + plan.items_off);
     a.units = reinterpret_cast<const WorkUnit *>(static_cast<const char *>(arena.p) + plan.units_off);
     a.partials = static_cast<double *>(partials.p);
-    a.out = static_cast<double *>(out.p);
-    a.out_dd = static_cast<double *>(out_dd.p);
+    a.out = static_cast<double *>(results.p);
+    a.out_dd = static_cast<double *>(results.p) + 2 * n_results;
     a.queue_head = static_cast<int *>(heads.p);
     a.n_units = static_cast<int32_t>(plan.units.size());
     a.n_items = static_cast<int32_t>(plan.items.size());
@@ -176,18 +207,29 @@ int upload(Exec &x) {
   int st;
   if ((st = x.arena.ensure(p.arena.size())) != PQP_OK) return st;
   if ((st = x.partials.ensure(std::max<int64_t>(p.n_partials, 1) * 4 * sizeof(double))) != PQP_OK) return st;
-  if ((st = x.out.ensure(n_out * 2 * sizeof(double))) != PQP_OK) return st;
-  if ((st = x.out_dd.ensure(n_out * 4 * sizeof(double))) != PQP_OK) return st;
+  if ((st = x.results.ensure(n_out * 6 * sizeof(double))) != PQP_OK) return st;
+  x.n_results = n_out;
+  const void *heads_before = x.heads.p;
   if ((st = x.heads.ensure(64 * sizeof(int))) != PQP_OK) return st;
-  PQP_HIP(hipMemcpyAsync(x.arena.p, p.arena.data(), p.arena.size(), hipMemcpyHostToDevice, x.stream));
-  PQP_HIP(hipMemsetAsync(x.heads.p, 0, 64 * sizeof(int), x.stream));
+  if (x.heads.p != heads_before) x.heads_clean = false;
+  // the previous use of the staging buffer (an upload or a download on this
+  // stream) has completed: every entry point synchronises before returning
+  if ((st = x.stage.ensure(p.arena.size())) != PQP_OK) return st;
+  std::memcpy(x.stage.p, p.arena.data(), p.arena.size());
+  PQP_HIP(hipMemcpyAsync(x.arena.p, x.stage.p, p.arena.size(), hipMemcpyHostToDevice, x.stream));
+  if (!x.heads_clean) {
+    // queue heads start at zero; the finalize kernel rewinds them after
+    // every launch that uses them
+    PQP_HIP(hipMemsetAsync(x.heads.p, 0, 64 * sizeof(int), x.stream));
+    x.heads_clean = true;
+  }
   return PQP_OK;
 }
 
 // Side streams (default priority) and fork/join events, created once per Exec.
 int ensure_side_streams(Exec &x) {
   if (x.ev_fork) return PQP_OK;
-  for (int i = 1; i < kBuckets; ++i) {
+  for (int i = 1; i < kMaxRanges; ++i) {
     PQP_HIP(hipStreamCreateWithFlags(&x.side[i], hipStreamNonBlocking));
     PQP_HIP(hipEventCreateWithFlags(&x.ev_join[i], hipEventDisableTiming));
   }
@@ -220,7 +262,7 @@ int enqueue(Exec &x, int mode, hipEvent_t ev_begin, hipEvent_t ev_end) {
         b = bucket_for_width(d.mcp, p.fp32, &mcp);
         PQP_HIP(launch_fwd(b, l));
       }
-      PQP_HIP(launch_finalize(p.fp32, a, k, 1, 0, x.stream));
+      PQP_HIP(launch_finalize(p.fp32, a, k, 1, 0, d.units, x.stream));
     }
   } else {
     const bool fixed = (x.flags & PQP_FLAG_STATIC) != 0;
@@ -273,10 +315,13 @@ int enqueue(Exec &x, int mode, hipEvent_t ev_begin, hipEvent_t ev_end) {
     }
     if (kernel_only && ev_end) PQP_HIP(hipEventRecord(ev_end, x.stream));
     const bool comp = !(x.flags & PQP_FLAG_PLAIN) || p.fp32;
-    if (p.minors)
+    if (p.all_direct && !p.minors) {
+      // every item was a single unit that wrote its own result
+    } else if (p.minors)
       PQP_HIP(launch_minors_finalize(a, n_items, x.stream));
     else
-      PQP_HIP(launch_finalize(comp, a, 0, n_items, static_cast<int>(p.ranges.size()), x.stream));
+      PQP_HIP(launch_finalize(comp, a, 0, n_items, static_cast<int>(p.ranges.size()),
+                              p.max_units, x.stream));
   }
   if (ev_end && !kernel_only) PQP_HIP(hipEventRecord(ev_end, x.stream));
   return PQP_OK;
@@ -321,8 +366,10 @@ int run_single(const PermInput &in, uint32_t flags, double *out2, double *out_dd
   opt.fp32_products = (flags & PQP_FLAG_FP32) != 0;
   opt.expand_cols = expand_mode(flags);
   const DeviceState &dev = device_state();
-  // enough waves to fill every CU several times over for large permanents
-  opt.min_units = static_cast<int64_t>(std::max(dev.cus, 1)) * kUnitsPerCu;
+  // waves per permanent: ~1.4 sqrt(T), at most kSingleUnitsPerCu per CU;
+  // a permanent of one wave writes its result from the sweep kernel
+  opt.auto_units_cap = static_cast<int64_t>(std::max(dev.cus, 1)) * kSingleUnitsPerCu;
+  opt.direct_out = (flags & PQP_FLAG_PLAIN) == 0;
   Plan plan;
   std::string err;
   int st = plan_build(std::vector<PermInput>{in}, opt, plan, err);
@@ -343,9 +390,11 @@ int run_single(const PermInput &in, uint32_t flags, double *out2, double *out_dd
   if ((st = enqueue(x, (flags & PQP_FLAG_PLAIN) ? PQP_RUN_PLAIN : PQP_RUN_EAGER, nullptr,
                     nullptr)) != PQP_OK)
     return st;
-  double res[6];
-  PQP_HIP(hipMemcpyAsync(res, x.out.p, 2 * sizeof(double), hipMemcpyDeviceToHost, x.stream));
-  PQP_HIP(hipMemcpyAsync(res + 2, x.out_dd.p, 4 * sizeof(double), hipMemcpyDeviceToHost, x.stream));
+  // one item: its 2 result doubles and 4 dd parts are contiguous
+  // (results = [out (2 per result)][out_dd (4 per result)], one result)
+  if ((st = x.stage.ensure(6 * sizeof(double))) != PQP_OK) return st;
+  double *res = static_cast<double *>(x.stage.p);
+  PQP_HIP(hipMemcpyAsync(res, x.results.p, 6 * sizeof(double), hipMemcpyDeviceToHost, x.stream));
   PQP_HIP(hipStreamSynchronize(x.stream));
   out2[0] = res[0];
   out2[1] = res[1];
@@ -381,8 +430,8 @@ void minor_inputs(const std::vector<PermInput> &inputs, std::vector<std::vector<
 // Complex results of the column-minor plan of x (host copy of x.out).
 int fetch_out(Exec &x, std::vector<double> &res) {
   res.assign(static_cast<size_t>(std::max<int64_t>(x.plan.n_out, 1)) * 2, 0.0);
-  PQP_HIP(hipMemcpyAsync(res.data(), x.out.p, res.size() * sizeof(double), hipMemcpyDeviceToHost,
-                         x.stream));
+  PQP_HIP(hipMemcpyAsync(res.data(), x.results.p, res.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, x.stream));
   PQP_HIP(hipStreamSynchronize(x.stream));
   return PQP_OK;
 }
@@ -470,7 +519,7 @@ int pqp_laplace_c128(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
   PlanOptions opt;
   opt.minors = true;
   const DeviceState &dev = device_state();
-  opt.min_units = static_cast<int64_t>(std::max(dev.cus, 1)) * kUnitsPerCu;
+  opt.auto_units_cap = static_cast<int64_t>(std::max(dev.cus, 1)) * kSingleUnitsPerCu;
   Plan plan;
   std::string err;
   int st = plan_build(std::vector<PermInput>{in}, opt, plan, err);
@@ -669,9 +718,10 @@ int pqp_batch_results(pqp_batch *b, double *out, double *out_dd) {
   if (!b->ran) return fail(PQP_ERR_INVALID, "batch has not been run");
   const size_t n = b->x.plan.items.size();
   PQP_HIP(hipStreamSynchronize(b->x.stream));
-  if (out && n) PQP_HIP(hipMemcpy(out, b->x.out.p, n * 2 * sizeof(double), hipMemcpyDeviceToHost));
+  if (out && n) PQP_HIP(hipMemcpy(out, b->x.results.p, n * 2 * sizeof(double), hipMemcpyDeviceToHost));
   if (out_dd && n)
-    PQP_HIP(hipMemcpy(out_dd, b->x.out_dd.p, n * 4 * sizeof(double), hipMemcpyDeviceToHost));
+    PQP_HIP(hipMemcpy(out_dd, static_cast<double *>(b->x.results.p) + 2 * b->x.n_results,
+                      n * 4 * sizeof(double), hipMemcpyDeviceToHost));
   return PQP_OK;
 }
 
@@ -770,20 +820,20 @@ void pqp_batch_destroy(pqp_batch *b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->xm.arena.release();
   b->xm.partials.release();
-  b->xm.out.release();
-  b->xm.out_dd.release();
+  b->xm.results.release();
+  b->xm.stage.release();
   b->xm.heads.release();
-  for (int i = 1; i < kBuckets; ++i) {
+  for (int i = 1; i < kMaxRanges; ++i) {
     if (b->xm.side[i]) (void)hipStreamDestroy(b->xm.side[i]);
     if (b->xm.ev_join[i]) (void)hipEventDestroy(b->xm.ev_join[i]);
   }
   if (b->xm.ev_fork) (void)hipEventDestroy(b->xm.ev_fork);
   b->x.arena.release();
   b->x.partials.release();
-  b->x.out.release();
-  b->x.out_dd.release();
+  b->x.results.release();
+  b->x.stage.release();
   b->x.heads.release();
-  for (int i = 1; i < kBuckets; ++i) {
+  for (int i = 1; i < kMaxRanges; ++i) {
     if (b->x.side[i]) (void)hipStreamDestroy(b->x.side[i]);
     if (b->x.ev_join[i]) (void)hipEventDestroy(b->x.ev_join[i]);
   }

@@ -59,6 +59,7 @@ hipError_t launch_minors(int bucket, const LaunchArgs &a, int first_unit, int n_
     case 5: return launch_minors_b5(a, first_unit, n_units, stream);
     case 6: return launch_minors_b6(a, first_unit, n_units, stream);
     case 7: return launch_minors_b7(a, first_unit, n_units, stream);
+    case 8: return launch_minors_b8(a, first_unit, n_units, stream);
     default: return hipErrorInvalidValue;
   }
 }

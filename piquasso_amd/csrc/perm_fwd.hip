@@ -234,11 +234,27 @@ __device__ __forceinline__ void sweep_unit(const LaunchArgs &a, int item, int lo
     }
   }
   if (lane == 0) {
-    double *p = a.partials + 4 * (it->partial_base + local);
-    p[0] = lane_sum.re.hi;
-    p[1] = lane_sum.re.lo;
-    p[2] = lane_sum.im.hi;
-    p[3] = lane_sum.im.lo;
+    if (flags & kDirectOut) {
+      // the item's only unit: scale by 2^-(n-1) and write the result
+      const int e = -(it->n - 1);
+      const double re_hi = ldexp(lane_sum.re.hi, e), re_lo = ldexp(lane_sum.re.lo, e);
+      const double im_hi = ldexp(lane_sum.im.hi, e), im_lo = ldexp(lane_sum.im.lo, e);
+      const int o = it->out_index;
+      a.out[2 * o] = re_hi + re_lo;
+      a.out[2 * o + 1] = im_hi + im_lo;
+      if (a.out_dd) {
+        a.out_dd[4 * o] = re_hi;
+        a.out_dd[4 * o + 1] = re_lo;
+        a.out_dd[4 * o + 2] = im_hi;
+        a.out_dd[4 * o + 3] = im_lo;
+      }
+    } else {
+      double *p = a.partials + 4 * (it->partial_base + local);
+      p[0] = lane_sum.re.hi;
+      p[1] = lane_sum.re.lo;
+      p[2] = lane_sum.im.hi;
+      p[3] = lane_sum.im.lo;
+    }
   }
 }
 
@@ -288,15 +304,22 @@ __attribute__((amdgpu_waves_per_eu(waves_for(HI, sizeof(R) == 4), 8))) void perm
   dispatch<LO, HI, R, COMP>(a, mcp, item, local);
 }
 
-// One wave per item: sum the item's partials in a fixed order, scale by
-// 2^-(n-1), write the complex128 result and its dd parts.  Block 0 also
-// rewinds the work-queue heads for the next launch of the same plan.
-template <bool COMP>
-__global__ __launch_bounds__(kWave) void perm_finalize_kernel(LaunchArgs a, int item0,
-                                                              int n_heads) {
+// One block of NT threads per item: thread t sums the item's partials
+// t, t + NT, ... (double-double), each wave reduces with a butterfly, and
+// the per-wave sums are added in wave order, so the result does not depend
+// on scheduling; then the 2^-(n-1) scale, the complex128 result and its dd
+// parts.  NT = 64 for batches of small items, 1024 when an item has
+// thousands of units.  Block 0 also rewinds the work-queue heads for the
+// next launch of the same plan.
+template <bool COMP, int NT>
+__global__ __launch_bounds__(NT) void perm_finalize_kernel(LaunchArgs a, int item0, int n_heads) {
+  constexpr int NW = NT / kWave;
+  __shared__ double part[NW][4];
   const int item = item0 + static_cast<int>(blockIdx.x);
-  const int lane = static_cast<int>(threadIdx.x);
-  if (blockIdx.x == 0 && lane < n_heads) a.queue_head[lane] = 0;
+  const int tid = static_cast<int>(threadIdx.x);
+  const int lane = tid & (kWave - 1);
+  const int wave = tid / kWave;
+  if (blockIdx.x == 0 && tid < n_heads) a.queue_head[tid] = 0;
   if (item >= a.n_items) return;
   const cptr<ItemDesc> it = as_const<ItemDesc>(a.items) + item;
   const int units = it->units;
@@ -304,9 +327,9 @@ __global__ __launch_bounds__(kWave) void perm_finalize_kernel(LaunchArgs a, int 
   cdd s = cdd_zero();
   if (it->n == 0) {
     // empty selection: permanent 1 by definition
-    if (lane == 0) s.re.hi = 1.0;
+    if (tid == 0) s.re.hi = 1.0;
   } else {
-    for (int k = lane; k < units; k += kWave) {
+    for (int k = tid; k < units; k += NT) {
       const double *p = a.partials + 4 * (base + k);
       if constexpr (COMP) {
         s = cdd_add(s, cdd{{p[0], p[1]}, {p[2], p[3]}});
@@ -325,7 +348,28 @@ __global__ __launch_bounds__(kWave) void perm_finalize_kernel(LaunchArgs a, int 
       s.im.hi += __shfl_xor(s.im.hi, off);
     }
   }
-  if (lane == 0) {
+  if constexpr (NW > 1) {
+    if (lane == 0) {
+      part[wave][0] = s.re.hi;
+      part[wave][1] = s.re.lo;
+      part[wave][2] = s.im.hi;
+      part[wave][3] = s.im.lo;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      s = cdd{{part[0][0], part[0][1]}, {part[0][2], part[0][3]}};
+      for (int w = 1; w < NW; ++w) {
+        const cdd o{{part[w][0], part[w][1]}, {part[w][2], part[w][3]}};
+        if constexpr (COMP) {
+          s = cdd_add(s, o);
+        } else {
+          s.re.hi += o.re.hi;
+          s.im.hi += o.im.hi;
+        }
+      }
+    }
+  }
+  if (tid == 0) {
     const int e = it->n > 0 ? -(it->n - 1) : 0;
     const double re_hi = ldexp(s.re.hi, e), re_lo = ldexp(s.re.lo, e);
     const double im_hi = ldexp(s.im.hi, e), im_lo = ldexp(s.im.lo, e);
@@ -429,14 +473,21 @@ hipError_t PQP_NAME(occupancy_fwd_b)(bool persistent, int *blocks) {
 
 #if PQP_BUCKET_ID == 0 && PQP_VARIANT == 0
 hipError_t launch_finalize(bool comp, const LaunchArgs &a, int item0, int n_items,
-                           int n_heads, hipStream_t stream) {
+                           int n_heads, int max_units, hipStream_t stream) {
   const int grid = n_items > 0 ? n_items : 1;
-  if (comp)
-    hipLaunchKernelGGL(perm_finalize_kernel<true>, dim3(grid), dim3(kWave), 0, stream, a,
+  const bool wide = max_units > 256;
+  if (comp && wide)
+    hipLaunchKernelGGL((perm_finalize_kernel<true, 1024>), dim3(grid), dim3(1024), 0, stream, a,
+                       item0, n_heads);
+  else if (comp)
+    hipLaunchKernelGGL((perm_finalize_kernel<true, kWave>), dim3(grid), dim3(kWave), 0, stream, a,
+                       item0, n_heads);
+  else if (wide)
+    hipLaunchKernelGGL((perm_finalize_kernel<false, 1024>), dim3(grid), dim3(1024), 0, stream, a,
                        item0, n_heads);
   else
-    hipLaunchKernelGGL(perm_finalize_kernel<false>, dim3(grid), dim3(kWave), 0, stream, a,
-                       item0, n_heads);
+    hipLaunchKernelGGL((perm_finalize_kernel<false, kWave>), dim3(grid), dim3(kWave), 0, stream,
+                       a, item0, n_heads);
   return hipGetLastError();
 }
 #endif

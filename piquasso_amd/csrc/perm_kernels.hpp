@@ -39,7 +39,9 @@ PQP_DECLARE_BUCKET(7)
 #undef PQP_DECLARE_BUCKET
 #undef PQP_DECLARE
 
-constexpr int kBuckets = 8;
+constexpr int kBuckets = 8;        // forward kernel width buckets
+constexpr int kMinorBuckets = 9;   // column-minor kernel buckets
+constexpr int kMaxRanges = 9;      // concurrent launches of one plan (one per bucket)
 
 // Column-minor kernels (perm_minors.hip), buckets 0..3 (widths up to 32).
 hipError_t launch_minors_b0(const LaunchArgs &a, int first_unit, int n_units, hipStream_t s);
@@ -50,6 +52,7 @@ hipError_t launch_minors_b4(const LaunchArgs &a, int first_unit, int n_units, hi
 hipError_t launch_minors_b5(const LaunchArgs &a, int first_unit, int n_units, hipStream_t s);
 hipError_t launch_minors_b6(const LaunchArgs &a, int first_unit, int n_units, hipStream_t s);
 hipError_t launch_minors_b7(const LaunchArgs &a, int first_unit, int n_units, hipStream_t s);
+hipError_t launch_minors_b8(const LaunchArgs &a, int first_unit, int n_units, hipStream_t s);
 hipError_t launch_minors(int bucket, const LaunchArgs &a, int first_unit, int n_units,
                          hipStream_t stream);
 hipError_t launch_minors_finalize(const LaunchArgs &a, int n_items, hipStream_t stream);
@@ -59,6 +62,6 @@ hipError_t occupancy_fwd(int bucket, bool persistent, bool fp32, bool comp, int 
 // Finalize items [item0, item0 + n_items); block 0 also rewinds n_heads
 // queue heads (pass 0 when a persistent launch may still be running).
 hipError_t launch_finalize(bool comp, const LaunchArgs &a, int item0, int n_items,
-                           int n_heads, hipStream_t stream);
+                           int n_heads, int max_units, hipStream_t stream);
 
 }  // namespace pqp

@@ -257,7 +257,7 @@ __device__ __forceinline__ void minors_dispatch(const LaunchArgs &a, int mcp, in
 // accumulators (about 12 m_c VGPRs); the general kernels also recompute
 // column powers.
 constexpr int minor_waves(int hi, bool ones) {
-  return hi <= 8 ? 4 : (ones && hi <= 16) ? 2 : 1;
+  return hi <= 8 ? 4 : (ones && hi <= 20) ? 2 : 1;
 }
 
 }  // namespace
@@ -318,26 +318,37 @@ __global__ __launch_bounds__(kWave) void perm_minors_finalize_kernel(LaunchArgs 
 #endif
 
 // ---------------------------------------------------------------------------
-// Instantiation per bucket (-DPQP_BUCKET_ID=0..7): widths 1-8, 9-16, 17-24,
-// 25-32; buckets 0-3 unit column multiplicities, 4-7 general.
+// Instantiation per bucket (-DPQP_BUCKET_ID=0..8): unit column
+// multiplicities in 0-4 (widths 1-8, 9-16, 17-18, 19-20, 21-32; up to 20
+// columns two waves per SIMD fit without spills), general ones in 5-8
+// (1-8, 9-16, 17-24, 25-32).
 // ---------------------------------------------------------------------------
 #ifndef PQP_BUCKET_ID
-#error "compile with -DPQP_BUCKET_ID=<0..7>"
+#error "compile with -DPQP_BUCKET_ID=<0..8>"
 #endif
-#if PQP_BUCKET_ID % 4 == 0
+#if PQP_BUCKET_ID == 0 || PQP_BUCKET_ID == 5
 #define PQP_LO 1
 #define PQP_HI 8
-#elif PQP_BUCKET_ID % 4 == 1
+#elif PQP_BUCKET_ID == 1 || PQP_BUCKET_ID == 6
 #define PQP_LO 9
 #define PQP_HI 16
-#elif PQP_BUCKET_ID % 4 == 2
+#elif PQP_BUCKET_ID == 2
+#define PQP_LO 17
+#define PQP_HI 18
+#elif PQP_BUCKET_ID == 3
+#define PQP_LO 19
+#define PQP_HI 20
+#elif PQP_BUCKET_ID == 4
+#define PQP_LO 21
+#define PQP_HI 32
+#elif PQP_BUCKET_ID == 7
 #define PQP_LO 17
 #define PQP_HI 24
 #else
 #define PQP_LO 25
 #define PQP_HI 32
 #endif
-#define PQP_ONES (PQP_BUCKET_ID < 4)
+#define PQP_ONES (PQP_BUCKET_ID < 5)
 
 #define PQP_PASTE_(base, k) base##k
 #define PQP_PASTE(base, k) PQP_PASTE_(base, k)

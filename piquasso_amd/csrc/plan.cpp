@@ -120,6 +120,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
   // can be split into a share of the requested total number of work units
   // proportional to their cost (load balance across uneven items).
   std::vector<double> item_cost(n_items, 0.0);
+  std::vector<double> item_terms(n_items, 0.0);
   double total_cost = 0.0;
   for (int64_t k = 0; k < n_items; ++k) {
     const PermInput &in = inputs[k];
@@ -151,6 +152,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
       if (rho > 0) T *= rho + 1;
     }
     item_cost[k] = opt.minors ? T * (16.0 * mc + 10.0) : T * (2.0 * mc + 4.0 * (sr - 1) + 8.0);
+    item_terms[k] = T;
     total_cost += item_cost[k];
   }
 
@@ -162,6 +164,12 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     d.out_index = static_cast<int32_t>(out_cursor);
     plan.info[k].out_offset = out_cursor;
     int64_t item_min_units = opt.min_units;
+    if (opt.auto_units_cap > 0 && item_terms[k] > 4096) {
+      // (up to 4096 terms one wave does the whole sum)
+      const double want = std::ceil(1.4 * std::sqrt(static_cast<double>(item_terms[k])));
+      item_min_units = std::max<int64_t>(
+          item_min_units, std::min<int64_t>(opt.auto_units_cap, static_cast<int64_t>(want)));
+    }
     if (opt.total_units > 0 && total_cost > 0.0) {
       const double share = item_cost[k] / total_cost * static_cast<double>(opt.total_units);
       item_min_units = std::max<int64_t>(item_min_units, static_cast<int64_t>(std::ceil(share)));
@@ -502,7 +510,8 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     d.Dt = Dt;
     d.cmax = cmax;
     d.flags = (cmax == 1 ? kColsAllOnes : 0) | (top_binary ? kTopBinary : 0) |
-              (opt.minors ? kMinors : 0) | (grouped ? kPowGrouped : 0);
+              (opt.minors ? kMinors : 0) | (grouped ? kPowGrouped : 0) |
+              (opt.direct_out && !opt.minors && d.units == 1 ? kDirectOut : 0);
     d.partial_base = partial_cursor;
     partial_cursor += static_cast<int64_t>(d.units) * (opt.minors ? mcp + 1 : 1);
     out_cursor += opt.minors ? mc + 1 : 1;
@@ -523,11 +532,21 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     plan.total_flops += inf.flops;
     plan.total_terms += T;
     item_units[k] = d.units;
-    // column-minor items: unit-multiplicity kernels in buckets 0-3, the
-    // general ones in 4-7
-    item_bucket[k] = opt.minors && cmax > 1 ? bucket + 4 : bucket;
+    // column-minor items: unit-multiplicity kernels in buckets 0-4 (widths
+    // 1-8, 9-16, 17-18, 19-20, 21-32), the general ones in 5-8 (1-8, 9-16,
+    // 17-24, 25-32)
+    if (opt.minors)
+      item_bucket[k] = cmax > 1 ? bucket + 5
+                                : (mc <= 8 ? 0 : mc <= 16 ? 1 : mc <= 18 ? 2 : mc <= 20 ? 3 : 4);
+    else
+      item_bucket[k] = bucket;
   }
   plan.n_partials = partial_cursor;
+  plan.all_direct = !plan.items.empty();
+  for (const ItemDesc &d : plan.items) {
+    plan.max_units = std::max(plan.max_units, d.units);
+    if (!(d.flags & kDirectOut)) plan.all_direct = false;
+  }
   plan.n_out = out_cursor;
   plan.minors = opt.minors;
 

@@ -40,6 +40,9 @@ struct PlanOptions {
   bool fp32_products = false;   // FP32 row sums and products (tables in float)
   int64_t target_inner = 0;     // requested inner states per lane (0 = cost model)
   int64_t min_units = 0;        // split every permanent into at least this many waves
+  bool direct_out = false;      // single-unit items write their result directly (kDirectOut)
+  int64_t auto_units_cap = 0;   // > 0: at least ~1.4 sqrt(T) waves per permanent, at most this many
+                                //      (measured optimum for one permanent on MI355X, n = 16..32)
   int expand_cols = 2;          // columns with multiplicity c > 1: 0 keep (power c),
                                 // 1 expand into c columns, 2 planner's choice (keep:
                                 // expansion widens the kernel and lowers occupancy)
@@ -72,6 +75,8 @@ struct Plan {
   struct Range { int bucket; int32_t first; int32_t count; double cost; };
   std::vector<Range> ranges;
   int64_t n_partials = 0;     // partial slots (4 doubles each)
+  int32_t max_units = 0;      // largest number of work units of one item
+  bool all_direct = false;    // every item writes its own result: no finalize launch
   int64_t n_out = 0;          // output slots (complex values)
   bool minors = false;        // column-minor items (perm_minors.hip kernels)
   bool fp32 = false;

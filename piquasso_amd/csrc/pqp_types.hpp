@@ -33,6 +33,7 @@ enum ItemFlags : int {
   kTopBinary = 2,     // every top digit has radix 2 (rho = 1): bit-level Gray
   kMinors = 4,        // column-minor item: perm(A; r, c - e_l) for every column l
   kPowGrouped = 8,    // column powers from prefix-product snapshots (masks at cpow_off)
+  kDirectOut = 16,    // one work unit: the unit writes the scaled result itself (no finalize)
 };
 
 // One permanent; read by the kernels with scalar loads (the item index is
