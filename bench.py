@@ -177,6 +177,8 @@ def main():
     ap.add_argument("--cpu-child", type=float, default=None)
     ap.add_argument("--profile", choices=["batch", "n32", "grad"], default=None,
                     help="profiling runs: only launch this workload's kernels --steps times")
+    ap.add_argument("--replay", choices=["eager", "graph"], default="eager",
+                    help="profiling runs: direct launches or hipGraph replays")
     ap.add_argument("--copy-calibration", type=int, default=None, metavar="MIB",
                     help="profiling runs: one device-to-device copy of MIB MiB (torch), to "
                          "calibrate FETCH_SIZE / WRITE_SIZE counters")
@@ -213,7 +215,7 @@ def main():
 
             cus = _capi.device_info()[1]
             b = pa.PermanentBatch([a32], [r32], [c32], persistent=False, total_units=128 * cus)
-        ms = b.time("eager", reps=args.steps)
+        ms = b.time(args.replay, reps=args.steps)
         print(json.dumps({"profile": args.profile, "ms": ms.tolist()}))
         return 0
     if args.gpus != 1:
