@@ -25,6 +25,7 @@ Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3]
 
 import argparse
 import json
+import math
 import os
 import subprocess
 import sys
@@ -52,14 +53,37 @@ def _load_batched():
     return mats, rows, cols, ref
 
 
-def _load_n32():
+def _load_extended(cid):
     with open(os.path.join(GOLDEN, "extended_vectors.json")) as f:
         cases = {c["id"]: c for c in json.load(f)["cases"]}
-    c = cases["large_single_n32"]
+    c = cases[cid]
     m = c["matrix"]
     a = (np.asarray(m["re"]) + 1j * np.asarray(m["im"])).reshape(m["shape"])
     dd = c["dd"]
     return a, np.asarray(c["rows"]), np.asarray(c["cols"]), complex(dd[0] + dd[1], dd[2] + dd[3])
+
+
+def _load_n32():
+    return _load_extended("large_single_n32")
+
+
+# the single-permanent configurations, timed through the drop-in entry
+SINGLE_CONFIGS = {
+    "plain_single_n12": "n = 12 block of a 24-mode Haar unitary, multiplicities 1",
+    "boson_sampling_m40_n20": "40-mode Haar unitary, 20 photons in modes 0..19, output pattern "
+                              "with multiplicities up to 4 (probability |perm|^2 / prod r! c!)",
+    "large_single_n20": "n = 20 i.i.d. N(0, 1/n) complex128",
+}
+
+
+def _median_call_us(fn, reps):
+    fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)) * 1e6
 
 
 # ------------------------------------------------------------ CPU leg --
@@ -88,8 +112,14 @@ def cpu_baseline_child(seconds: float) -> None:
         flops += W.glynn_flops(rows[k], cols[k])
         done += 1
     dt = time.perf_counter() - t0
+    singles = {}
+    for cid in SINGLE_CONFIGS:
+        a, r, c, _ = _load_extended(cid)
+        reps = 50 if a.shape[0] <= 12 else 5
+        singles[cid] = _median_call_us(lambda: fn(a, r, c), reps)
     print(json.dumps({"kind": kind, "items": done, "seconds": dt, "flops": flops,
-                      "threads": int(os.environ.get("OMP_THREAD_LIMIT", "0") or 0)}))
+                      "threads": int(os.environ.get("OMP_THREAD_LIMIT", "0") or 0),
+                      "singles_us": singles}))
 
 
 def run_cpu_baseline(seconds: float):
@@ -120,6 +150,7 @@ def run_cpu_baseline(seconds: float):
                    f"{rec['seconds']:.1f} s, reference pybind permanent(), "
                    f"OMP_THREAD_LIMIT={threads}"),
         "gflops": rec["flops"] / rec["seconds"] / 1e9 if rec["seconds"] > 0 else None,
+        "singles_us": rec.get("singles_us"),
     }
 
 
@@ -272,6 +303,29 @@ def main():
 
     grad = _gradient_leg(pa, W, reps=max(3, min(args.steps, 10)))
 
+    # single permanents through the drop-in entry (host buffers in, result
+    # out: planning, upload, sweep, download); the reference's CPU time for
+    # the same call comes from the CPU leg
+    singles = {}
+    for cid, what in SINGLE_CONFIGS.items():
+        a, r, c, ref_c = _load_extended(cid)
+        reps = 50 if a.shape[0] <= 12 else 20
+        got = complex(pa.permanent(a, r, c))
+        rec_s = {"what": what, "gpu_call_us": _median_call_us(lambda: pa.permanent(a, r, c), reps),
+                 "rel_err": abs(got - ref_c) / abs(ref_c)}
+        if cid == "boson_sampling_m40_n20":
+            norm = float(np.prod([math.factorial(int(k)) for k in r]) *
+                         np.prod([math.factorial(int(k)) for k in c]))
+            rec_s["probability"] = abs(got) ** 2 / norm
+        if cid == "large_single_n20":
+            v32, _ = pa.permanent_ex(a, r, c, mode="fp32")
+            rec_s["fp32_products"] = {
+                "gpu_call_us": _median_call_us(lambda: pa.permanent_ex(a, r, c, mode="fp32"), reps),
+                "rel_err": abs(v32 - ref_c) / abs(ref_c)}
+        if cpu and cpu.get("singles_us"):
+            rec_s["reference_cpu_us"] = cpu["singles_us"].get(cid)
+        singles[cid] = rec_s
+
     line = {
         "metric": METRIC,
         "value": value,
@@ -305,6 +359,7 @@ def main():
         "speedup_vs_plain": float(np.median(plain_ms)) / float(np.median(graph_ms)),
         "n32": n32,
         "gradient": grad,
+        "single_permanents": singles,
         "roofline": {
             "bound": "mfma",
             "pipe": "FP64 VALU: the sweep is a product of row sums, not a contraction; "

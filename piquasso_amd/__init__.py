@@ -10,14 +10,17 @@ PermanentBatch / permanent_batch     many uneven permanents in one persistent la
 permanent_laplace(matrix, rows, cols) drop-in for piquasso._math.permanent.permanent_laplace
 permanent_grad / permanent_grad_batch d perm / d A (grad_perm, any shape)
 torch_permanent(A, rows, cols)       differentiable permanent (torch.autograd.Function)
+output_probabilities(U, s, outputs)  boson-sampling probabilities of many outputs in one batch
 """
 
 from ._math.permanent import permanent, permanent_laplace
 from .engine import PermanentBatch, permanent_batch, permanent_ex
 from .grad import PermanentFunction, permanent_grad, permanent_grad_batch, torch_permanent
+from .probabilities import output_amplitudes, output_probabilities
 
 __all__ = [
     "permanent", "permanent_laplace", "permanent_ex", "PermanentBatch", "permanent_batch",
     "permanent_grad", "permanent_grad_batch", "PermanentFunction", "torch_permanent",
+    "output_amplitudes", "output_probabilities",
 ]
 __version__ = "0.1.0"

@@ -232,3 +232,21 @@ def test_boson_sampling_probability(gpu, oracle):
     got = complex(pa.permanent(u, rows, cols))
     ref = oracle.perm_dd(u, rows, cols)
     assert rel_err(got, ref) < 1e-11
+
+
+def test_batched_output_probabilities(gpu, reference_vectors):
+    """output_probabilities (one batch for all output patterns) reproduces the
+    reference's probability known answers and sums to 1 over every output
+    pattern of a unitary (3 photons in 5 modes: 35 patterns)."""
+    for case in reference_vectors["probability"]:
+        u = _case_matrix(case)
+        p = pa.output_probabilities(u, case["input"], [case["output"]])
+        assert abs(p[0] - case["expected"]) < 1e-8, case["id"]
+    rng = np.random.default_rng(9)
+    u = W.haar_unitary(5, rng)
+    s = [1, 0, 1, 1, 0]
+    outs = [t for t in itertools.product(range(4), repeat=5) if sum(t) == 3]
+    p = pa.output_probabilities(u, s, outs)
+    assert len(outs) == 35 and abs(p.sum() - 1.0) < 1e-12
+    # a pattern with the wrong photon number has probability 0
+    assert pa.output_probabilities(u, s, [[1, 1, 1, 1, 0]])[0] == 0.0
