@@ -243,9 +243,11 @@ def main():
         else:
             a32, r32, c32, _ = _load_n32()
             from piquasso_amd import _capi
+            from piquasso_amd.tune import best_config
 
-            cus = _capi.device_info()[1]
-            b = pa.PermanentBatch([a32], [r32], [c32], persistent=False, total_units=128 * cus)
+            _, cus, arch = _capi.device_info()
+            cfg32 = best_config(1, cus, arch) or {"persistent": False, "total_units": 128 * cus}
+            b = pa.PermanentBatch([a32], [r32], [c32], **cfg32)
         ms = b.time(args.replay, reps=args.steps)
         print(json.dumps({"profile": args.profile, "ms": ms.tolist()}))
         return 0
@@ -285,9 +287,13 @@ def main():
 
     # single large permanent, n = 32 (2^31 Gray terms)
     a32, r32, c32, ref32 = _load_n32()
-    # one large permanent: 128 work units per CU, one block per unit (the
-    # fastest setting measured for n = 32 on MI355X)
-    b32 = pa.PermanentBatch([a32], [r32], [c32], persistent=False, total_units=128 * cus)
+    # one large permanent: the autotuned launch for this device, else 128
+    # work units per CU with one block per unit (the fastest setting
+    # measured for n = 32 on MI355X)
+    from piquasso_amd.tune import best_config
+
+    cfg32 = best_config(1, cus, arch) or {"persistent": False, "total_units": 128 * cus}
+    b32 = pa.PermanentBatch([a32], [r32], [c32], **cfg32)
     b32.run("eager")
     v32 = b32.results()[0]
     t32 = b32.time("kernel", reps=5)

@@ -76,8 +76,9 @@ class PermanentBatch:
         "fp32" (FP32 row sums and products, double-double accumulation).
     persistent:
         True: one persistent work-stealing launch per column-width bucket;
-        False (default, faster on MI355X for uneven batches): one block per
-        work unit, units in decreasing cost order.
+        False: one block per work unit, units in decreasing cost order;
+        None (default): the autotuned choice for this device
+        (:func:`piquasso_amd.tune.best_config`), else False.
     target_inner:
         inner Gray states per lane (0 lets the planner's cost model choose;
         the autotuner in :mod:`piquasso_amd.tune` picks it per workload).
@@ -87,7 +88,8 @@ class PermanentBatch:
         sum is raised to the power c).
     total_units:
         wave work units spread over the batch in proportion to each
-        permanent's cost (0: 32 per compute unit).
+        permanent's cost (0: the autotuned count for this device, else 32
+        per compute unit).
     gradient:
         also evaluate d perm / d A of every item on each run (column-minor
         sweeps after the permanents, on the same stream and in the same
@@ -100,7 +102,7 @@ class PermanentBatch:
         rows: Sequence,
         cols: Sequence,
         precision: str = "fp64",
-        persistent: bool = False,
+        persistent=None,
         target_inner: int = 0,
         expand_cols=None,
         total_units: int = 0,
@@ -112,6 +114,18 @@ class PermanentBatch:
             raise ValueError("precision must be 'fp64' or 'fp32'")
         self.count = len(matrices)
         self.precision = precision
+        if persistent is None or not total_units:
+            from .tune import best_config
+
+            try:
+                _, cus, arch = _capi.device_info()
+                tuned = best_config(len(matrices), cus, arch)
+            except RuntimeError:  # no device: creation fails below with the C library's error
+                tuned = {}
+            if persistent is None:
+                persistent = bool(tuned.get("persistent", False))
+            if not total_units:
+                total_units = int(tuned.get("total_units", 0))
         self.gradient = bool(gradient)
         self._keep = []
         mats, rws, cls = [], [], []

@@ -59,7 +59,7 @@ def sweep(mats, rows, cols, grid: dict, reps: int = 5, mode: str = "kernel", log
 def _workloads():
     from . import workloads as W
 
-    mats, rows, cols = W.batched_uneven(count=4096, seed=4)
+    mats, rows, cols = W.batched_uneven(count=4096, seed=4)  # the batched benchmark workload
     a32, r32, c32 = W.large_single(32, seed=3)
     a24, r24, c24 = W.large_single(24, seed=3)
     g_m, g_r, g_c = W.gradient_batch(count=256, seed=5)
@@ -72,12 +72,30 @@ def _workloads():
 
 
 GRIDS = {
-    "batched_uneven": {"total_units": [4096, 8192, 12288, 24576, 49152],
-                       "expand_cols": [None, False], "persistent": [True, False]},
-    "single_n32": {"total_units": [4096, 8192, 16384, 32768], "persistent": [False, True]},
-    "single_n24": {"total_units": [2048, 4096, 8192, 16384], "persistent": [False, True]},
-    "uniform_n12_20": {"total_units": [4096, 12288, 24576], "persistent": [True, False]},
+    "batched_uneven": {"total_units": [4096, 8192, 12288, 16384], "persistent": [False, True]},
+    "single_n32": {"total_units": [16384, 32768, 65536], "persistent": [False, True]},
+    "single_n24": {"total_units": [1024, 2048, 4096, 8192], "persistent": [False, True]},
+    "uniform_n12_20": {"total_units": [2048, 4096, 8192], "persistent": [False, True]},
 }
+
+
+def best_config(count: int, cus: int, arch: str, path: str = None) -> dict:
+    """Launch settings for a batch of `count` permanents from the tuned table
+    of this device (PermanentBatch kwargs: total_units, persistent), or {}
+    when no table was written for it.  One permanent uses the single-n32
+    entry, a batch the batched_uneven entry; unit counts scale with the CU
+    count of the table's device."""
+    table = load_tuning(path or DEFAULT_TABLE)
+    if not table or not arch.split(":")[0] == str(table.get("device", "")).split(":")[0]:
+        return {}
+    name = "single_n32" if count == 1 else "batched_uneven"
+    rec = table.get("workloads", {}).get(name)
+    if not rec:
+        return {}
+    cfg = dict(rec["best"])
+    if "total_units" in cfg and table.get("cus"):
+        cfg["total_units"] = int(round(cfg["total_units"] / table["cus"] * cus))
+    return {k: v for k, v in cfg.items() if k in ("total_units", "persistent")}
 
 
 def tune(out: str = DEFAULT_TABLE, reps: int = 5, only=None, log=print) -> dict:
