@@ -152,7 +152,7 @@ __device__ __forceinline__ void sweep_unit(const LaunchArgs &a, int item, int lo
   const bool active = u < U;
   if (!active) u = U - 1;
 
-  const cptr<R> B2 = as_const<R>(a.arena, it->b2_off);
+  const cptr<R> Bin = as_const<R>(a.arena, it->b2_off);  // inner rows of B
   const cptr<InnerEntry> tab = as_const<InnerEntry>(a.arena, it->step_off);
   const cptr<int32_t> cpow = as_const<int32_t>(a.arena, it->cpow_off + 24);
   PowState ps{0u, 0u, 0u, 0};
@@ -172,12 +172,13 @@ __device__ __forceinline__ void sweep_unit(const LaunchArgs &a, int item, int lo
   if (!active) lane_w = 0.0;
 
   // Inner sweep in blocks of kDdFold states: FP64 run sums inside a block,
-  // folded into the double-double accumulators between blocks.  The row
-  // added after state v (its first PF columns; the scalar registers hold at
-  // most 16 complex values) is fetched before state v's product, and the
-  // table entry of state v + 1 one state ahead, so scalar-load latency
-  // hides behind the product's VALU work.  The update after the last state
-  // uses the padding entry (row 0) and is never read.
+  // folded into the double-double accumulators between blocks.  The inner
+  // row of B added (times +-2) after state v (its first PF columns; the
+  // scalar registers hold at most 16 complex values) is fetched before state
+  // v's product, and the table entry of state v + 1 one state ahead, so
+  // scalar-load latency hides behind the product's VALU work.  The update
+  // after the last state uses the padding entry (0 x row 0) and is never
+  // read.
   constexpr int PF = MCP < 16 ? MCP : 16;
   dd accr{0.0, 0.0}, acci{0.0, 0.0};
   InnerEntry e = inner_entry(tab, 0);
@@ -185,7 +186,7 @@ __device__ __forceinline__ void sweep_unit(const LaunchArgs &a, int item, int lo
     const int vend = v0 + kDdFold < S ? v0 + kDdFold : S;
     double sr = 0.0, si = 0.0;
     for (int v = v0; v < vend; ++v) {
-      const cptr<R> row = B2 + static_cast<int64_t>(e.row) * (2 * MCP);
+      const cptr<R> row = Bin + static_cast<int64_t>(e.row) * (2 * MCP);
       R ur[PF], ui[PF];
 #pragma unroll
       for (int j = 0; j < PF; ++j) {
@@ -197,15 +198,17 @@ __device__ __forceinline__ void sweep_unit(const LaunchArgs &a, int item, int lo
       column_product<MCP, R>(cr, ci, pr, pi, flags, cmax, ps, cpow);
       sr = __builtin_fma(e.coef, static_cast<double>(pr), sr);
       si = __builtin_fma(e.coef, static_cast<double>(pi), si);
+      // row sums += (+-2) x inner row: exact doubling, one rounding (FMA)
+      const R s2 = in_vgpr(static_cast<R>(e.sign2));
 #pragma unroll
       for (int j = 0; j < PF; ++j) {
-        cr[j] += ur[j];
-        ci[j] += ui[j];
+        cr[j] = fmaR(s2, ur[j], cr[j]);
+        ci[j] = fmaR(s2, ui[j], ci[j]);
       }
 #pragma unroll
       for (int j = PF; j < MCP; ++j) {
-        cr[j] += row[2 * j];
-        ci[j] += row[2 * j + 1];
+        cr[j] = fmaR(s2, static_cast<R>(row[2 * j]), cr[j]);
+        ci[j] = fmaR(s2, static_cast<R>(row[2 * j + 1]), ci[j]);
       }
       e = en;
     }

@@ -52,7 +52,7 @@ __device__ __forceinline__ void minors_unit(const LaunchArgs &a, int item, int l
   const bool active = u < U;
   if (!active) u = U - 1;
 
-  const cptr<R> B2 = as_const<R>(a.arena, it->b2_off);
+  const cptr<R> Bin = as_const<R>(a.arena, it->b2_off);  // inner rows of B
   const cptr<InnerEntry> tab = as_const<InnerEntry>(a.arena, it->step_off);
   const cptr<int32_t> cmult = as_const<int32_t>(a.arena, it->cmul_off);
 
@@ -72,7 +72,7 @@ __device__ __forceinline__ void minors_unit(const LaunchArgs &a, int item, int l
   constexpr int PF = MCP < 8 ? MCP : 8;  // row columns fetched ahead of the products
   InnerEntry e = inner_entry(tab, 0);
   for (int v = 0; v < S; ++v) {
-    const cptr<R> row = B2 + static_cast<int64_t>(e.row) * (2 * MCP);
+    const cptr<R> row = Bin + static_cast<int64_t>(e.row) * (2 * MCP);
     R ur[PF], ui[PF];
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
@@ -208,16 +208,18 @@ __device__ __forceinline__ void minors_unit(const LaunchArgs &a, int item, int l
       aci[MCP] = __builtin_fma(cf, qi, aci[MCP]);
     }
 
-    // (after the last state this adds the padding row 0; never read)
+    // row sums += (+-2) x inner row (after the last state the padding
+    // entry adds 0 x row 0; never read)
+    const R s2 = in_vgpr(static_cast<R>(e.sign2));
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
-      cr[j] += ur[j];
-      ci[j] += ui[j];
+      cr[j] = fmaR(s2, ur[j], cr[j]);
+      ci[j] = fmaR(s2, ui[j], ci[j]);
     }
 #pragma unroll
     for (int j = PF; j < MCP; ++j) {
-      cr[j] += row[2 * j];
-      ci[j] += row[2 * j + 1];
+      cr[j] = fmaR(s2, static_cast<R>(row[2 * j]), cr[j]);
+      ci[j] = fmaR(s2, static_cast<R>(row[2 * j + 1]), ci[j]);
     }
     e = en;
   }

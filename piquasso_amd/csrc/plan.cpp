@@ -367,7 +367,8 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     // --- tables ---
     const int rows_B = 1 + L + Dt;
     d.b_off = arena_alloc(plan.arena, int64_t(rows_B) * mcp * 2 * rsz);
-    d.b2_off = arena_alloc(plan.arena, int64_t(2 * std::max(L, 1)) * mcp * 2 * rsz);
+    // inner rows of B: the sweep adds +-2 times one of them per step
+    d.b2_off = d.b_off + int64_t(mcp) * 2 * rsz;
     auto put = [&](int64_t off, int64_t row, int col, double re, double im) {
       int64_t e = (row * mcp + col) * 2;
       if (opt.fp32_products) {
@@ -405,14 +406,6 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
         put(d.b_off, r, j, re, im);
       }
     }
-    // +-2B rows of the inner digits (doubling is exact)
-    for (int t = 0; t < L; ++t)
-      for (int j = 0; j < mc; ++j) {
-        double re, im;
-        entry(digits[inner[t]].src_row, colsrc[j], re, im);
-        put(d.b2_off, 2 * t, j, 2.0 * re, 2.0 * im);
-        put(d.b2_off, 2 * t + 1, j, -2.0 * re, -2.0 * im);
-      }
     // inner start coefficients alpha_k = rho_k (all inner g = 0)
     d.alpha_off = arena_alloc(plan.arena, int64_t(std::max(L, 1)) * 8);
     for (int t = 0; t < L; ++t)
@@ -428,27 +421,27 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
       std::vector<int> sd, sv;
       gray_steps(radix, sd, sv);
       // Interleaved inner table, one 16-byte entry per inner state v:
-      // {double coefficient, uint32 row of the +-2B table added after state v,
-      // uint32 0}.  S_in + 1 entries: the kernels read one entry ahead, and
-      // the entries past the last step are padding (row 0, coefficient 0).
+      // {double coefficient, float sign2, uint32 inner row}: after state v
+      // the row sums get sign2 x (inner row of B).  S_in + 1 entries: the
+      // kernels read one entry ahead, and the entries past the last step are
+      // padding (sign2 0, row 0, coefficient 0).
       tb.step_off = arena_alloc(plan.arena, (S_in + 1) * 16);
       tb.coef_off = tb.step_off;
       char *tab = plan.arena.data() + tb.step_off;
-      auto set_entry = [&](int64_t v, double coef, uint32_t row) {
-        std::memcpy(tab + 16 * v, &coef, 8);
-        std::memcpy(tab + 16 * v + 8, &row, 4);
-      };
+      const double one = 1.0;
+      std::memcpy(tab, &one, 8);
       std::vector<int> g(L, 0);
       int64_t w = 1;
-      set_entry(0, 1.0, 0);
       for (size_t s = 0; s < sd.size(); ++s) {
         const int t = sd[s];
         const int prev = g[t];
         g[t] = sv[s];
-        // moving g up subtracts 2 b_t from the row sums (row 2t+1 = -2B),
-        // moving it down adds 2 b_t (row 2t)
-        const uint32_t row = static_cast<uint32_t>(2 * t + (sv[s] > prev ? 1 : 0));
-        std::memcpy(tab + 16 * s + 8, &row, 4);
+        // moving g up subtracts 2 b_t from the row sums, moving it down adds
+        // 2 b_t (doubling is exact)
+        const float sign2 = sv[s] > prev ? -2.0f : 2.0f;
+        const uint32_t row = static_cast<uint32_t>(t);
+        std::memcpy(tab + 16 * s + 8, &sign2, 4);
+        std::memcpy(tab + 16 * s + 12, &row, 4);
         w = w / binom64(profile[t], prev) * binom64(profile[t], g[t]);
         const double sgn = ((s + 1) & 1) ? -1.0 : 1.0;
         const double coef = sgn * static_cast<double>(w);

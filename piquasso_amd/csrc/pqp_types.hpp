@@ -41,7 +41,8 @@ enum ItemFlags : int {
 struct alignas(16) ItemDesc {
   int64_t b_off;        // B: (1 + L + Dt) rows x mcp complex, row 0 = prepended row,
                         //    rows 1..L inner digits, rows L+1.. top digits (unscaled)
-  int64_t b2_off;       // inner +-2B: 2L rows x mcp complex (row 2k = +2B_k, 2k+1 = -2B_k)
+  int64_t b2_off;       // the inner rows of B (row 1 of B): the row added at an inner
+                        //    step is (+-2) x row k of this block
   int64_t step_off;     // inner table, S_in + 1 entries of InnerEntry (shared by items
                         //    with the same inner multiplicity profile)
   int64_t coef_off;     // the same table (kept for the layout)
@@ -67,13 +68,14 @@ struct alignas(16) ItemDesc {
 };
 static_assert(sizeof(ItemDesc) == 128, "ItemDesc layout");
 
-// Inner Gray state v: its coefficient (-1)^v prod_inner C(rho, g) and the
-// row of the +-2B table added after it (entries past the last step are
-// padding: row 0, coefficient 0).
+// Inner Gray state v: its coefficient (-1)^v prod_inner C(rho, g), and the
+// update applied after it: row sums += sign2 * (inner row `row` of B), with
+// sign2 = +2 when the digit moves down, -2 when it moves up (entries past
+// the last step are padding: sign2 0, row 0, coefficient 0).
 struct alignas(16) InnerEntry {
   double coef;
+  float sign2;
   uint32_t row;
-  uint32_t pad;
 };
 static_assert(sizeof(InnerEntry) == 16, "InnerEntry layout");
 

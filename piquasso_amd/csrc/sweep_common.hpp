@@ -78,8 +78,8 @@ __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirs
 __device__ __forceinline__ InnerEntry inner_entry(cptr<InnerEntry> tab, int v) {
   InnerEntry e;
   e.coef = tab[v].coef;
+  e.sign2 = tab[v].sign2;
   e.row = tab[v].row;
-  e.pad = 0u;
   return e;
 }
 

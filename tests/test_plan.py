@@ -41,16 +41,18 @@ def plan_image(a, rows, cols, target_inner=0, min_units=0, flags=0):
     return buf
 
 
-INNER = np.dtype([("coef", "<f8"), ("row", "<u4"), ("pad", "<u4")])
+INNER = np.dtype([("coef", "<f8"), ("sign2", "<f4"), ("row", "<u4")])
 
 
 def inner_table(buf, d, S):
     """The interleaved inner table: per inner state v its coefficient and the
-    +-2B row added after it (S + 1 entries, padding past the last step)."""
+    update after it, sign2 (+-2) x inner row of B (S + 1 entries, padding
+    past the last step)."""
     off = int(d["step_off"])
     t = np.frombuffer(buf[off: off + 16 * (S + 1)].tobytes(), dtype=INNER)
-    assert t["row"][S - 1] == 0 and t["row"][S] == 0 and t["coef"][S] == 0.0
-    return t["coef"][:S].copy(), t["row"][: max(S - 1, 0)].copy()
+    assert t["sign2"][S - 1] == 0 and t["sign2"][S] == 0 and t["coef"][S] == 0.0
+    assert set(np.abs(t["sign2"][: S - 1]).tolist()) <= {2.0}
+    return t["coef"][:S].copy(), t[: max(S - 1, 0)].copy()
 
 
 def emulate(buf):
@@ -64,7 +66,7 @@ def emulate(buf):
         return np.frombuffer(buf[off: off + 8 * count].tobytes(), dtype=np.float64)
 
     B = f64(int(d["b_off"]), (1 + L + Dt) * mcp * 2).view(np.complex128).reshape(1 + L + Dt, mcp)
-    B2 = f64(int(d["b2_off"]), 2 * max(L, 1) * mcp * 2).view(np.complex128).reshape(-1, mcp)
+    assert int(d["b2_off"]) == int(d["b_off"]) + 16 * mcp  # inner rows of B
     alpha = f64(int(d["alpha_off"]), max(L, 1))
     coef, steps = inner_table(buf, d, S)
     top = f64(int(d["top_off"]), 4 * max(Dt, 1)).reshape(-1, 4)
@@ -108,7 +110,7 @@ def emulate(buf):
                     p *= np.prod(cs[: cpow[rr - 2]])
             acc += coef[v] * p
             if v + 1 < S:
-                cs = cs + B2[steps[v]]
+                cs = cs + float(steps[v]["sign2"]) * B[1 + int(steps[v]["row"])]
         total += lane_w * acc
     return total / 2.0 ** (int(d["n"]) - 1)
 
@@ -186,7 +188,7 @@ def emulate_minors(buf, c):
         return np.frombuffer(buf[off: off + 8 * count].tobytes(), dtype=np.float64)
 
     B = f64(int(d["b_off"]), (1 + L + Dt) * mcp * 2).view(np.complex128).reshape(1 + L + Dt, mcp)
-    B2 = f64(int(d["b2_off"]), 2 * max(L, 1) * mcp * 2).view(np.complex128).reshape(-1, mcp)
+    assert int(d["b2_off"]) == int(d["b_off"]) + 16 * mcp  # inner rows of B
     alpha = f64(int(d["alpha_off"]), max(L, 1))
     coef, steps = inner_table(buf, d, S)
     top = f64(int(d["top_off"]), 4 * max(Dt, 1)).reshape(-1, 4)
@@ -213,7 +215,7 @@ def emulate_minors(buf, c):
                 e = cm - (np.arange(mc) == l)
                 acc[l] += lane_w * coef[v] * np.prod(x ** e)
             if v + 1 < S:
-                cs = cs + B2[steps[v]]
+                cs = cs + float(steps[v]["sign2"]) * B[1 + int(steps[v]["row"])]
     acc /= 2.0 ** (int(d["n"]) - 1)
     order = sorted(np.flatnonzero(np.asarray(c) > 0), key=lambda j: -c[j])
     out = np.full(len(c), acc[mc])
