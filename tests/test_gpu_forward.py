@@ -250,3 +250,32 @@ def test_batched_output_probabilities(gpu, reference_vectors):
     assert len(outs) == 35 and abs(p.sum() - 1.0) < 1e-12
     # a pattern with the wrong photon number has probability 0
     assert pa.output_probabilities(u, s, [[1, 1, 1, 1, 0]])[0] == 0.0
+
+
+@pytest.mark.parametrize("cols", [
+    [3, 3, 2, 1],            # multiplicity drops 0, 1, 1, 0: grouped powers
+    [4, 1, 1, 2],            # a drop of 2 after sorting (4, 2, 1, 1)
+    [4, 1, 1],               # a drop of 3: third snapshot mask
+    [5, 1],                  # a drop of 4: count form
+    [2] * 6 + [1] * 3,       # one drop, nine columns (bucket 9-16)
+    [1, 3, 1, 1, 2, 1, 1, 1, 1, 1, 2, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1],  # 26 columns: count form
+])
+def test_column_power_forms(gpu, oracle, cols):
+    """Column multiplicities through every product form of the sweep kernel:
+    prefix snapshots with one, two or three masks, the count form for larger
+    drops and for kernels wider than the snapshot form's limit."""
+    from conftest import glynn_abs_sum
+
+    rng = np.random.default_rng(len(cols) * 7 + sum(cols))
+    c = np.array(cols)
+    n = int(c.sum())
+    m_r = min(n, 6)
+    r = np.bincount(np.concatenate([np.arange(m_r), rng.integers(0, m_r, n - m_r)]), minlength=m_r)
+    a = (rng.standard_normal((m_r, len(c))) + 1j * rng.standard_normal((m_r, len(c)))) / np.sqrt(2 * len(c))
+    ref = oracle.perm_dd(a, r, c)
+    got = complex(pa.permanent(a, r, c))
+    tol = max(TOL * abs(ref), 4 * (n + 2) * 2.0 ** -53 * glynn_abs_sum(a, r, c))
+    assert abs(got - ref) <= tol, (cols, got, ref)
+    # the same item inside a batch (another launch path) gives the same value
+    vals = pa.permanent_batch([a, a], [r, r], [c, c])
+    assert abs(vals[0] - ref) <= tol and vals[0] == vals[1]
