@@ -158,6 +158,19 @@ int pqp_laplace_c128(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
                      const int32_t *rows, const int32_t *cols, double *out);
 
 /*
+ * Column minors of `count` problems from one launch: outs[k] (2 * m_c[k]
+ * doubles) receives what pqp_laplace_c128 writes for problem k.  The batched
+ * form of permanent_laplace (piquasso/_math/permanent.cpp:43-58,82-84) for
+ * callers that need many at once; the reference calls the single entry once
+ * per shot and photon step of Clifford & Clifford sampling
+ * (piquasso/_simulators/passive/sampling.py:723-749), and running the shots
+ * in lockstep turns each step into one such batch.
+ */
+int pqp_laplace_batch_c128(int64_t count, const double *const *matrices, const int64_t *m_r,
+                           const int64_t *m_c, const int64_t *ld, const int32_t *const *rows,
+                           const int32_t *const *cols, double *const *outs);
+
+/*
  * Gradients of `count` permanents: grads[k] (m_r[k] x m_c[k] complex,
  * row-major, interleaved) = d perm(A_k; r_k, c_k) / d (A_k)_ij
  * = r_i c_j perm(A_k; r_k - e_i, c_k - e_j) (0 where r_i or c_j is 0), the

@@ -74,6 +74,11 @@ SIGNATURES = {
     "pqp_laplace_c128": (
         _c.c_int, [_dp, _c.c_int64, _c.c_int64, _c.c_int64, _i32p, _i32p, _dp],
     ),
+    "pqp_laplace_batch_c128": (
+        _c.c_int,
+        [_c.c_int64, _c.POINTER(_dp), _i64p, _i64p, _i64p, _c.POINTER(_i32p),
+         _c.POINTER(_i32p), _c.POINTER(_dp)],
+    ),
     "pqp_grad_batch_c128": (
         _c.c_int,
         [_c.c_int64, _c.POINTER(_dp), _i64p, _i64p, _i64p, _c.POINTER(_i32p),

@@ -48,7 +48,7 @@ import numpy as np
 
 from .. import _capi
 
-__all__ = ["permanent", "permanent_laplace", "resolve_dtype"]
+__all__ = ["permanent", "permanent_laplace", "permanent_laplace_batch", "resolve_dtype"]
 
 
 def resolve_dtype(matrix):
@@ -128,3 +128,51 @@ def permanent_laplace(matrix, rows, cols):
     )
     _capi.check(st)
     return (out[0::2] + 1j * out[1::2]).astype(dtype)
+
+
+def permanent_laplace_batch(matrices, rows, cols):
+    """``[permanent_laplace(m, r, c) for m, r, c in zip(matrices, rows, cols)]``
+    evaluated in one launch (``pqp_laplace_batch_c128``): every problem's
+    column minors come from the same batched sweep.  Each entry of the
+    returned list is a complex128 array with the value and length
+    ``permanent_laplace`` gives (``[1]`` for a degenerate problem).  Inputs
+    are computed in FP64 whatever their dtype."""
+    mats, rs, cs = list(matrices), list(rows), list(cols)
+    if not (len(mats) == len(rs) == len(cs)):
+        raise ValueError("permanent_laplace_batch(): matrices, rows and cols differ in length")
+    out = [None] * len(mats)
+    keep_a, keep_r, keep_c, keep_o, idx = [], [], [], [], []
+    for k, (m, r, c) in enumerate(zip(mats, rs, cs)):
+        a = np.ascontiguousarray(np.asarray(m, dtype=np.complex128))
+        if a.ndim != 2:
+            raise ValueError(f"permanent_laplace_batch(): item {k} is not 2-D (shape {a.shape})")
+        m_r, m_c = a.shape
+        rr = _multiplicities(r, "rows", m_r)
+        cc = _multiplicities(c, "cols", m_c)
+        if m_r == 0 or m_c == 0 or int(rr.sum()) == 0 or int(cc.sum()) == 0:
+            out[k] = np.ones(1, dtype=np.complex128)
+            continue
+        keep_a.append(a)
+        keep_r.append(rr)
+        keep_c.append(cc)
+        keep_o.append(np.zeros(2 * m_c, dtype=np.float64))
+        idx.append(k)
+    n = len(idx)
+    if n:
+        dp = ctypes.POINTER(ctypes.c_double)
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        mats_p = (dp * n)(*[a.view(np.float64).ctypes.data_as(dp) for a in keep_a])
+        rows_p = (i32p * n)(*[r.ctypes.data_as(i32p) for r in keep_r])
+        cols_p = (i32p * n)(*[c.ctypes.data_as(i32p) for c in keep_c])
+        outs_p = (dp * n)(*[o.ctypes.data_as(dp) for o in keep_o])
+        m_r = np.array([a.shape[0] for a in keep_a], dtype=np.int64)
+        m_c = np.array([a.shape[1] for a in keep_a], dtype=np.int64)
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        st = _capi.lib().pqp_laplace_batch_c128(
+            n, mats_p, m_r.ctypes.data_as(i64p), m_c.ctypes.data_as(i64p),
+            m_c.ctypes.data_as(i64p), rows_p, cols_p, outs_p,
+        )
+        _capi.check(st)
+        for k, o in zip(idx, keep_o):
+            out[k] = o[0::2] + 1j * o[1::2]
+    return out

@@ -554,6 +554,63 @@ int pqp_laplace_c128(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
   return PQP_OK;
 }
 
+int pqp_laplace_batch_c128(int64_t count, const double *const *matrices, const int64_t *m_r,
+                           const int64_t *m_c, const int64_t *ld, const int32_t *const *rows,
+                           const int32_t *const *cols, double *const *outs) {
+  if (count < 0 || (count > 0 && (!matrices || !m_r || !m_c || !ld || !rows || !cols || !outs)))
+    return fail(PQP_ERR_INVALID, "invalid laplace batch arguments");
+  if (count > INT32_MAX) return fail(PQP_ERR_RANGE, "too many items");
+  std::vector<PermInput> inputs;
+  inputs.reserve(static_cast<size_t>(count));
+  for (int64_t k = 0; k < count; ++k) {
+    if (!outs[k] && m_c[k] > 0) return fail(PQP_ERR_INVALID, "item " + std::to_string(k) + ": out is null");
+    inputs.push_back(PermInput{matrices[k], false, m_r[k], m_c[k], ld[k], rows[k], cols[k]});
+  }
+  if (count == 0) return PQP_OK;
+  PlanOptions opt;
+  opt.minors = true;
+  const DeviceState &dev = device_state();
+  const int64_t cus = static_cast<int64_t>(std::max(dev.cus, 1));
+  // about kBatchUnitsPerCu waves per CU, shared in proportion to cost
+  opt.total_units = cus * kBatchUnitsPerCu;
+  Plan plan;
+  std::string err;
+  int st = plan_build(inputs, opt, plan, err);
+  if (st != PQP_OK) return fail(st, err);
+  if ((st = require_device()) != PQP_OK) return st;
+  SingleState &S = single_state();
+  std::lock_guard<std::mutex> lock(S.mu);
+  Exec &x = S.x;
+  if (!S.init) {
+    PQP_HIP(hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking));
+    x.own_stream = true;
+    S.init = true;
+  }
+  x.plan = std::move(plan);
+  index_units(x);
+  x.flags = PQP_FLAG_STATIC;
+  if ((st = upload(x)) != PQP_OK) return st;
+  if ((st = enqueue(x, PQP_RUN_EAGER, nullptr, nullptr)) != PQP_OK) return st;
+  std::vector<double> res;
+  if ((st = fetch_out(x, res)) != PQP_OK) return st;
+  std::vector<int64_t> slot_of;
+  for (int64_t k = 0; k < count; ++k) {
+    const ItemInfo &inf = x.plan.info[k];
+    const int64_t mc = static_cast<int64_t>(inf.colsrc.size());
+    // columns of zero multiplicity get the uncut product (slot mc), as in
+    // pqp_laplace_c128
+    slot_of.assign(static_cast<size_t>(m_c[k]), mc);
+    for (int64_t l = 0; l < mc; ++l) slot_of[static_cast<size_t>(inf.colsrc[l])] = l;
+    double *o = outs[k];
+    for (int64_t j = 0; j < m_c[k]; ++j) {
+      const double *v = res.data() + 2 * (inf.out_offset + slot_of[static_cast<size_t>(j)]);
+      o[2 * j] = v[0];
+      o[2 * j + 1] = v[1];
+    }
+  }
+  return PQP_OK;
+}
+
 int pqp_grad_batch_c128(int64_t count, const double *const *matrices, const int64_t *m_r,
                         const int64_t *m_c, const int64_t *ld, const int32_t *const *rows,
                         const int32_t *const *cols, double *perms, double *const *grads) {

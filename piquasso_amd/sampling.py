@@ -1,0 +1,365 @@
+"""Clifford & Clifford boson sampling with the shots in lockstep on the GPU.
+
+piquasso samples a linear-optical circuit with algorithm B of Clifford &
+Clifford (arXiv:1706.01260), generalised to multiply occupied inputs: photon
+by photon, the next output mode is drawn from a distribution built from the
+column minors perm(U_sub; sample, input - e_l) of the interferometer
+(piquasso/_simulators/passive/sampling.py:197-236,711-753).  The reference
+runs the shots one after the other and calls ``permanent_laplace`` once per
+shot and photon.  Here every shot advances one photon per round and each
+round's minors of all shots come from ONE launch of the column-minor kernels
+(``pqp_laplace_batch_c128``); the bookkeeping around them (input growth,
+distributions, draws) is vectorised over the shots.
+
+The samples equal the reference's for equal seeds.  Each shot draws from its
+own ``numpy.random.default_rng(seed + shot)`` (sampling.py:178-193) in the
+reference's order, so the batching cannot change them:
+
+* the reference's ``rng.choice(len(pool))`` is drawn as
+  ``rng.integers(0, len(pool), dtype=np.int64)`` and its
+  ``rng.choice(np.arange(d), p=pmf)`` as one ``rng.random()`` located in the
+  normalised cumulative sum (searchsorted, side='right') — the draws numpy's
+  ``Generator.choice`` makes for those calls (checked draw for draw by the
+  tests);
+* ``reject_condition`` (uniform losses: the reference's closure draws from
+  the simulator's generator, simulation_steps.py:352-360) is called for every
+  shot before the rounds start, in the reference's shot-major order, as many
+  times as the reference calls it (once per photon).  Where that count is not
+  fixed in advance (postselection restarts a shot), the shots run one at a
+  time instead.
+
+Supported: ideal and uniformly lossy sampling (sampling.py:208-236),
+postselection (:239-325), uniform partial distinguishability
+(:328-419) and non-uniform losses through the 2d-mode unitary dilation
+(:110-146,756-784).  Not supported: postselection together with partial
+distinguishability (:422-529), which raises ``NotImplementedError``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from typing import Callable, Optional, Sequence
+
+import numpy as np
+
+from . import _capi
+
+__all__ = ["generate_samples", "generate_lossy_samples", "expanded_lossy_matrix",
+           "SampleGenerationError"]
+
+
+class SampleGenerationError(RuntimeError):
+    """A postselected shot needed more trials than allowed (the reference
+    raises its ``InvalidSimulation`` there, sampling.py:275-283)."""
+
+
+def _seed_of(config, seed_sequence):
+    if seed_sequence is not None:
+        return int(seed_sequence)
+    if config is not None and getattr(config, "seed_sequence", None) is not None:
+        return int(config.seed_sequence)
+    # piquasso's Config draws a fresh seed the same way when none is given
+    return int.from_bytes(os.urandom(8), "big")
+
+
+class _MinorsBatch:
+    """Column minors of many (rows, cols) multiplicity patterns over ONE
+    interferometer through ``pqp_laplace_batch_c128``: problem s is
+    perm(U; rows_s, cols_s - e_l) for every column l; zero multiplicities
+    are compacted away by the planner, so no sub-matrix is copied."""
+
+    def __init__(self, u: np.ndarray):
+        self.u = np.ascontiguousarray(u, dtype=np.complex128)
+        self.d = self.u.shape[0]
+        self.lib = _capi.lib()
+        self._dp = ctypes.POINTER(ctypes.c_double)
+        self._i32p = ctypes.POINTER(ctypes.c_int32)
+        self._i64p = ctypes.POINTER(ctypes.c_int64)
+
+    def __call__(self, rows: np.ndarray, cols: np.ndarray) -> np.ndarray:
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        count, d = rows.shape
+        out = np.zeros((count, 2 * d), dtype=np.float64)
+        if count == 0:
+            return out.view(np.complex128)
+        step_i = np.arange(count, dtype=np.uint64) * np.uint64(4 * d)
+        mats = np.full(count, self.u.ctypes.data, dtype=np.uint64)
+        rp = np.uint64(rows.ctypes.data) + step_i
+        cp = np.uint64(cols.ctypes.data) + step_i
+        op = np.uint64(out.ctypes.data) + np.arange(count, dtype=np.uint64) * np.uint64(16 * d)
+        dims = np.full(count, d, dtype=np.int64)
+        pp = ctypes.POINTER(self._dp)
+        st = self.lib.pqp_laplace_batch_c128(
+            count, mats.ctypes.data_as(pp), dims.ctypes.data_as(self._i64p),
+            dims.ctypes.data_as(self._i64p), dims.ctypes.data_as(self._i64p),
+            rp.ctypes.data_as(ctypes.POINTER(self._i32p)),
+            cp.ctypes.data_as(ctypes.POINTER(self._i32p)), op.ctypes.data_as(pp),
+        )
+        _capi.check(st)
+        return out.view(np.complex128)
+
+
+class _Lockstep:
+    """S shots of Clifford & Clifford sampling advanced one photon per round.
+
+    Per shot: ``pool`` (the input particles not yet placed, by mode, in the
+    reference's order), ``cur`` (the input grown so far), ``smp`` (the
+    output so far), ``step`` (photon iterations done of ``n``).  ``reject``
+    maps an array of shot indices to one rejection decision each, in call
+    order; ``postselect`` = (modes, photons, max_trials) or None.
+    """
+
+    def __init__(self, minors: _MinorsBatch, rngs, pools, reject, postselect=None):
+        self.minors = minors
+        self.u_t = np.ascontiguousarray(minors.u.T)
+        d = minors.d
+        S = len(pools)
+        self.rngs = rngs
+        self.n = np.array([len(p) for p in pools], dtype=np.int64)
+        width = max(int(self.n.max()) if S else 0, 1)
+        self.pool0 = np.full((S, width), -1, dtype=np.int64)
+        for s, p in enumerate(pools):
+            self.pool0[s, : len(p)] = p
+        self.reject = reject
+        self.ps = postselect
+        self.cur = np.zeros((S, d), dtype=np.int32)
+        self.smp = np.zeros((S, d), dtype=np.int32)
+        self.pool = self.pool0.copy()
+        self.plen = self.n.copy()
+        self.step = np.zeros(S, dtype=np.int64)
+        if postselect is not None:
+            modes, photons, _ = postselect
+            self.ps_modes = np.asarray(modes, dtype=np.int64)
+            self.ps_photons = np.asarray(photons, dtype=np.int64)
+            self.left = np.tile(self.ps_photons, (S, 1))
+            self.needed = np.full(S, int(self.ps_photons.sum()), dtype=np.int64)
+            self.trials = np.ones(S, dtype=np.int64)
+            self.mode_slot = np.full(d, -1, dtype=np.int64)
+            # the reference's tuple(modes).index(mode): the first listed slot
+            for q in range(len(self.ps_modes) - 1, -1, -1):
+                self.mode_slot[self.ps_modes[q]] = q
+        self.rounds = 0
+        self.problems = 0
+
+    def _restart(self, shots):
+        self.trials[shots] += 1
+        over = shots[self.trials[shots] > self.ps[2]]
+        if over.size:
+            raise SampleGenerationError(
+                "Too many trials during sample generation. The specified postselection "
+                "criteria may be very highly unlikely. Aborting. To increase the limit, "
+                "set Config.max_sample_generation_trials to a higher value. Current value: "
+                f"{self.ps[2]}.")
+        self.cur[shots] = 0
+        self.smp[shots] = 0
+        self.pool[shots] = self.pool0[shots]
+        self.plen[shots] = self.n[shots]
+        self.step[shots] = 0
+        self.left[shots] = self.ps_photons
+        self.needed[shots] = int(self.ps_photons.sum())
+
+    def run(self) -> np.ndarray:
+        while True:
+            live = np.flatnonzero(self.step < self.n)
+            if self.reject is not None:
+                # rejected iterations place no particle (sampling.py:222-224);
+                # each shot takes decisions until one keeps its particle
+                settled = []
+                pending = live
+                while pending.size:
+                    rej = self.reject(pending)
+                    settled.append(pending[~rej])
+                    skipped = pending[rej]
+                    self.step[skipped] += 1
+                    pending = skipped[self.step[skipped] < self.n[skipped]]
+                live = np.sort(np.concatenate(settled)) if settled else live
+            if live.size == 0:
+                return self.smp
+            # grow the input by a uniformly chosen pool particle (sampling.py:197-205)
+            for s in live:
+                i = int(self.rngs[s].integers(0, self.plen[s], dtype=np.int64))
+                L = int(self.plen[s])
+                self.cur[s, self.pool[s, i]] += 1
+                self.pool[s, i: L - 1] = self.pool[s, i + 1: L]
+                self.plen[s] = L - 1
+            # distribution of the next output mode (sampling.py:711-749)
+            cur = self.cur[live]
+            lap = self.minors(self.smp[live], cur)
+            w = np.where(cur > 0, cur * lap, 0)
+            amp = w @ self.u_t
+            pmf = amp.real ** 2 + amp.imag ** 2
+            norm = pmf.sum(axis=1, keepdims=True)
+            if not np.all(np.isfinite(norm)) or np.any(norm <= 0):
+                raise ValueError("probabilities contain NaN")
+            cdf = np.cumsum(pmf / norm, axis=1)
+            cdf /= cdf[:, -1:]
+            u = np.array([self.rngs[s].random() for s in live])
+            mode = (cdf <= u[:, None]).sum(axis=1)
+            mode = np.minimum(mode, cdf.shape[1] - 1)
+            self.smp[live, mode] += 1
+            self.step[live] += 1
+            self.rounds += 1
+            self.problems += live.size
+            if self.ps is not None:
+                slot = self.mode_slot[mode]
+                hit = slot >= 0
+                if hit.any():
+                    hs = live[hit]
+                    self.needed[hs] -= 1
+                    self.left[hs, slot[hit]] -= 1
+                over = np.zeros(live.size, dtype=bool)
+                if hit.any():
+                    over[hit] = (self.left[live[hit]] < 0).any(axis=1)
+                # too few photons left for the postselected modes (sampling.py:314-315)
+                short = ~over & (self.needed[live] > self.n[live] - self.step[live])
+                fail = live[over | short]
+                if fail.size:
+                    self._restart(fail)
+
+
+def _first_quantized(occupation) -> np.ndarray:
+    """Mode of every particle, modes ascending (piquasso/_math/indices.py:105-114)."""
+    return np.repeat(np.arange(len(occupation)), np.asarray(occupation, dtype=np.int64))
+
+
+def _split_particles(occupation, overlap, rng):
+    """sampling.py:371-399: K of n_j particles stay indistinguishable with
+    weight C(n_j, K) x^K (1 - x)^(n_j - K) K!."""
+    indist = np.zeros(len(occupation), dtype=np.int64)
+    dist = np.zeros(len(occupation), dtype=np.int64)
+    for j, nj in enumerate(occupation):
+        nj = int(nj)
+        if nj == 0:
+            continue
+        w = np.array([math.comb(nj, q) * overlap ** q * (1.0 - overlap) ** (nj - q) * math.factorial(q)
+                      for q in range(nj + 1)], dtype=float)
+        q = int(rng.choice(nj + 1, p=w / w.sum()))
+        indist[j] = q
+        dist[j] = nj - q
+    return indist, dist
+
+
+def _distinguishable(u, particles, rng, reject_one):
+    """sampling.py:402-419: each distinguishable particle leaves by |U_{:,j}|^2."""
+    d = len(particles)
+    out = np.zeros(d, dtype=np.int64)
+    for mode, count in enumerate(particles):
+        p = np.abs(u[:, mode]) ** 2
+        p = p / p.sum()
+        for _ in range(int(count)):
+            target = rng.choice(d, p=p)
+            if not reject_one():
+                out[target] += 1
+    return out
+
+
+def generate_samples(input, shots: int, interferometer,
+                     reject_condition: Optional[Callable[[], bool]] = None,
+                     postselect_data: Sequence = ((), (), 1000),
+                     uniform_particle_overlap: Optional[float] = None,
+                     config=None, *, seed_sequence: Optional[int] = None):
+    """Samples of piquasso's ``generate_samples``
+    (piquasso/_simulators/passive/sampling.py:34-107), same argument names
+    minus ``calculate_permanent_laplace`` (the minors always come from the
+    batched GPU kernels).  ``config`` is anything with a ``seed_sequence``
+    attribute (piquasso's ``Config``), or give ``seed_sequence`` directly;
+    ``reject_condition`` None means no particle is ever rejected.  Returns a
+    list of ``shots`` tuples of ints, in shot order."""
+    occ = np.asarray(input, dtype=np.int64).reshape(-1)
+    if (occ < 0).any():
+        raise ValueError("occupation numbers must be non-negative")
+    u = np.asarray(interferometer, dtype=np.complex128)
+    d = len(occ)
+    if u.shape != (d, d):
+        raise ValueError(f"interferometer must be {d} x {d}, got {u.shape}")
+    if shots < 0:
+        raise ValueError("shots must be non-negative")
+    n = int(occ.sum())
+    seed = _seed_of(config, seed_sequence)
+    ps_modes, ps_photons, max_trials = postselect_data
+    postselected = len(ps_modes) > 0
+    if postselected and uniform_particle_overlap is not None:
+        raise NotImplementedError(
+            "postselection together with partial distinguishability is not supported")
+    minors = _MinorsBatch(u)
+    if shots == 0:
+        return []
+    rngs = [np.random.default_rng(seed=seed + s) for s in range(shots)]
+
+    if postselected and reject_condition is not None:
+        # the number of reject_condition calls depends on the restarts: keep
+        # the reference's global call order by running the shots one by one
+        out = []
+        for s in range(shots):
+            live_reject = lambda idx: np.array([bool(reject_condition()) for _ in idx])  # noqa: E731
+            eng = _Lockstep(minors, [rngs[s]], [_first_quantized(occ)], live_reject,
+                            (ps_modes, ps_photons, max_trials))
+            smp = eng.run()[0]
+            out.append(tuple(int(v) for v in np.delete(smp, list(ps_modes))))
+        return out
+
+    decisions = None
+    pointer = np.zeros(shots, dtype=np.int64)
+    if reject_condition is not None and n > 0:
+        # n calls per shot, shot after shot: the reference's order
+        decisions = np.array([[bool(reject_condition()) for _ in range(n)] for _ in range(shots)],
+                             dtype=bool).reshape(shots, n)
+
+    def reject_one(s):
+        if decisions is None:
+            return False
+        v = decisions[s, pointer[s]]
+        pointer[s] += 1
+        return bool(v)
+
+    def reject_many(shot_idx):
+        return np.array([reject_one(s) for s in shot_idx], dtype=bool)
+
+    if uniform_particle_overlap is None:
+        pools = [_first_quantized(occ)] * shots
+        dist_out = None
+    else:
+        pools, dist_out = [], np.zeros((shots, d), dtype=np.int64)
+        for s in range(shots):
+            indist, dist = _split_particles(occ, float(uniform_particle_overlap), rngs[s])
+            dist_out[s] = _distinguishable(u, dist, rngs[s], lambda s=s: reject_one(s))
+            pools.append(_first_quantized(indist))
+    eng = _Lockstep(minors, rngs, pools, reject_many if decisions is not None else None,
+                    (ps_modes, ps_photons, max_trials) if postselected else None)
+    smp = eng.run().astype(np.int64)
+    if dist_out is not None:
+        smp = dist_out + smp
+    if postselected:
+        smp = np.delete(smp, list(ps_modes), axis=1)
+    return [tuple(int(v) for v in row) for row in smp]
+
+
+def expanded_lossy_matrix(interferometer) -> np.ndarray:
+    """The 2d x 2d unitary dilation of a lossy interferometer T = V diag(s) W
+    (numpy's SVD): [[V, 0], [0, I]] [[diag s, diag t], [diag t, diag s]]
+    [[W, 0], [0, I]] with t = sqrt(1 - s^2) (sampling.py:756-784)."""
+    v, s, w = np.linalg.svd(np.asarray(interferometer, dtype=np.complex128))
+    d = len(s)
+    z = np.zeros((d, d))
+    eye = np.eye(d)
+    t = np.sqrt(1.0 - s ** 2)
+    return (np.block([[v, z], [z, eye]])
+            @ np.block([[np.diag(s), np.diag(t)], [np.diag(t), np.diag(s)]])
+            @ np.block([[w, z], [z, eye]]))
+
+
+def generate_lossy_samples(input, shots: int, interferometer,
+                           postselect_data: Sequence = ((), (), 1000),
+                           config=None, *, seed_sequence: Optional[int] = None):
+    """Samples of piquasso's ``generate_lossy_samples`` (sampling.py:110-146):
+    the dilation sampled with empty auxiliary inputs, first d modes kept."""
+    occ = np.asarray(input, dtype=np.int64).reshape(-1)
+    d = len(occ)
+    big = expanded_lossy_matrix(interferometer)
+    full = np.concatenate([occ, np.zeros(d, dtype=np.int64)])
+    out = generate_samples(full, shots, big, postselect_data=postselect_data, config=config,
+                           seed_sequence=seed_sequence)
+    return [s[:d] for s in out]
