@@ -13,6 +13,10 @@ Printed as ONE JSON line (rank 0):
                   permanent (speedup_vs_plain = value / plain rate)
   n32             one n = 32 permanent (2^31 Gray terms): time, % of FP64 peak
   gradient        gradients/s of 256 permanents n ~ U{12..20} (d perm / dA)
+  sampling        Clifford & Clifford boson samples/s (40 modes, 20 photons,
+                  1024 shots in lockstep: one batched column-minor launch per
+                  photon round), checked shot for shot against the reference's
+                  own permanent_laplace driven sequentially on the host
   rel_err_max     vs the double-double oracle values committed with the inputs
   roofline        the sweep kernels: algorithmic FLOP / kernel time vs 78.6 TFLOP/s;
                   traffic = HBM bytes per evaluation from the committed rocprofv3
@@ -41,6 +45,9 @@ PROFILES = os.path.join(ROOT, "profiles")
 TRAFFIC_FILE = os.path.join(PROFILES, "hbm_traffic_batched_uneven.txt")
 METRIC = "permanents/s batched n\u2208[8,24] and n=32 %FP64 peak, vs plain, rel.err\u22641e-10"
 GOLDEN = os.path.join(ROOT, "tests", "golden")
+SAMPLING_SHOTS = 1024
+SAMPLING_SEED = 1234
+SAMPLING_CPU_SECONDS = 8.0
 
 
 def _load_batched():
@@ -119,7 +126,29 @@ def cpu_baseline_child(seconds: float) -> None:
         singles[cid] = _median_call_us(lambda: fn(a, r, c), reps)
     print(json.dumps({"kind": kind, "items": done, "seconds": dt, "flops": flops,
                       "threads": int(os.environ.get("OMP_THREAD_LIMIT", "0") or 0),
-                      "singles_us": singles}))
+                      "singles_us": singles, "sampling": _cpu_sampling(SAMPLING_CPU_SECONDS)}))
+
+
+def _cpu_sampling(seconds: float):
+    """The sampling workload on the host: the reference's permanent_laplace
+    (oracle/_ref) called shot after shot by the oracle's restatement of the
+    reference's sampler loop, shots 0, 1, ... until `seconds` are used."""
+    from oracle import oracle as O
+    from oracle import sampling as S
+    from piquasso_amd import workloads as W
+
+    u, occ = W.sampling()
+    try:
+        O.ref_lib()
+        lap, kind = O.ref_laplace, "reference"
+    except O.OracleUnavailable:
+        lap, kind = O.laplace_dd, "port"
+    samples = []
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        samples += S.generate_samples(occ, 1, u, SAMPLING_SEED + len(samples), laplace=lap)
+    return {"kind": kind, "shots": len(samples), "seconds": time.perf_counter() - t0,
+            "samples": samples}
 
 
 def run_cpu_baseline(seconds: float):
@@ -151,6 +180,7 @@ def run_cpu_baseline(seconds: float):
                    f"OMP_THREAD_LIMIT={threads}"),
         "gflops": rec["flops"] / rec["seconds"] / 1e9 if rec["seconds"] > 0 else None,
         "singles_us": rec.get("singles_us"),
+        "sampling": rec.get("sampling"),
     }
 
 
@@ -198,6 +228,46 @@ def _gradient_leg(pa, W, reps: int):
     }
 
 
+def _sampling_leg(pa, W, cpu):
+    """Clifford & Clifford sampling of 20 photons in 40 modes, 1024 shots in
+    lockstep (20 rounds, each one batched column-minor launch over all
+    shots); samples/s over the whole call, host bookkeeping included.  The
+    shots the CPU leg also drew must come out identical."""
+    from piquasso_amd import sampling as PS
+
+    u, occ = W.sampling()
+    pa.generate_samples(occ, 16, u, seed_sequence=SAMPLING_SEED)  # warm-up
+    t0 = time.perf_counter()
+    samples = pa.generate_samples(occ, SAMPLING_SHOTS, u, seed_sequence=SAMPLING_SEED)
+    wall = time.perf_counter() - t0
+    st = dict(PS.last_run)
+    rec = {
+        "samples_per_s": SAMPLING_SHOTS / wall,
+        "seconds": wall,
+        "shots": SAMPLING_SHOTS,
+        "photons": int(occ.sum()),
+        "modes": len(occ),
+        "rounds": st["rounds"],
+        "minor_problems": st["problems"],
+        "minors_seconds": st["minors_seconds"],
+        "minors_tflops": st["minors_flops"] / st["minors_seconds"] / 1e12 if st["minors_seconds"] else None,
+        "what": "boson sampling (Clifford & Clifford B), 40-mode Haar unitary, one photon in each of "
+                "modes 0..19, shots in lockstep: one pqp_laplace_batch_c128 call per photon round",
+    }
+    cs = (cpu or {}).get("sampling")
+    if cs and cs.get("shots"):
+        k = cs["shots"]
+        rec["reference_cpu"] = {
+            "samples_per_s": k / cs["seconds"],
+            "kind": cs["kind"],
+            "cores": cpu.get("cores"),
+            "sample": f"shots 0..{k - 1} one after another in {cs['seconds']:.1f} s",
+        }
+        rec["shots_identical_to_reference"] = [tuple(x) for x in cs["samples"]] == samples[:k]
+        rec["shots_compared"] = k
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,7 +276,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-child", type=float, default=None)
-    ap.add_argument("--profile", choices=["batch", "n32", "grad"], default=None,
+    ap.add_argument("--profile", choices=["batch", "n32", "grad", "sampling"], default=None,
                     help="profiling runs: only launch this workload's kernels --steps times")
     ap.add_argument("--replay", choices=["eager", "graph"], default="eager",
                     help="profiling runs: direct launches or hipGraph replays")
@@ -232,6 +302,14 @@ def main():
     if args.profile is not None:
         import piquasso_amd as pa
 
+        if args.profile == "sampling":
+            from piquasso_amd import workloads as W
+
+            u, occ = W.sampling()
+            for _ in range(args.steps):
+                pa.generate_samples(occ, SAMPLING_SHOTS, u, seed_sequence=SAMPLING_SEED)
+            print(json.dumps({"profile": "sampling", "last_run": pa.sampling.last_run}))
+            return 0
         if args.profile == "batch":
             mats, rows, cols, _ = _load_batched()
             b = pa.PermanentBatch(mats, rows, cols)
@@ -308,6 +386,7 @@ def main():
     }
 
     grad = _gradient_leg(pa, W, reps=max(3, min(args.steps, 10)))
+    sampling = _sampling_leg(pa, W, cpu)
 
     # single permanents through the drop-in entry (host buffers in, result
     # out: planning, upload, sweep, download); the reference's CPU time for
@@ -365,6 +444,7 @@ def main():
         "speedup_vs_plain": float(np.median(plain_ms)) / float(np.median(graph_ms)),
         "n32": n32,
         "gradient": grad,
+        "sampling": sampling,
         "single_permanents": singles,
         "roofline": {
             "bound": "mfma",
@@ -380,7 +460,7 @@ def main():
             "flop_model": "sum over items of T * (2 m_c + 6 (n - 1) + 4), T = Gray terms",
             "units": stats["units"],
         },
-        "cpu_baseline": cpu,
+        "cpu_baseline": {k: v for k, v in cpu.items() if k != "sampling"} if cpu else cpu,
     }
     print(json.dumps(line))
     return 0

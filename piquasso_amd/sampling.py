@@ -40,6 +40,7 @@ from __future__ import annotations
 import ctypes
 import math
 import os
+import time
 from typing import Callable, Optional, Sequence
 
 import numpy as np
@@ -47,7 +48,12 @@ import numpy as np
 from . import _capi
 
 __all__ = ["generate_samples", "generate_lossy_samples", "expanded_lossy_matrix",
-           "SampleGenerationError"]
+           "SampleGenerationError", "last_run"]
+
+# Counters of the most recent generate_samples call in this process: rounds,
+# column-minor problems, their algorithmic FLOP, and the seconds spent in the
+# batched minors call (planning, upload, kernels, download) and in total.
+last_run: dict = {}
 
 
 class SampleGenerationError(RuntimeError):
@@ -77,8 +83,16 @@ class _MinorsBatch:
         self._dp = ctypes.POINTER(ctypes.c_double)
         self._i32p = ctypes.POINTER(ctypes.c_int32)
         self._i64p = ctypes.POINTER(ctypes.c_int64)
+        self.seconds = 0.0
 
     def __call__(self, rows: np.ndarray, cols: np.ndarray) -> np.ndarray:
+        t0 = time.perf_counter()
+        try:
+            return self._run(rows, cols)
+        finally:
+            self.seconds += time.perf_counter() - t0
+
+    def _run(self, rows: np.ndarray, cols: np.ndarray) -> np.ndarray:
         rows = np.ascontiguousarray(rows, dtype=np.int32)
         cols = np.ascontiguousarray(cols, dtype=np.int32)
         count, d = rows.shape
@@ -143,6 +157,7 @@ class _Lockstep:
                 self.mode_slot[self.ps_modes[q]] = q
         self.rounds = 0
         self.problems = 0
+        self.flops = 0.0
 
     def _restart(self, shots):
         self.trials[shots] += 1
@@ -187,7 +202,9 @@ class _Lockstep:
                 self.plen[s] = L - 1
             # distribution of the next output mode (sampling.py:711-749)
             cur = self.cur[live]
-            lap = self.minors(self.smp[live], cur)
+            rows = self.smp[live]
+            lap = self.minors(rows, cur)
+            self.flops += _minors_flops(rows, cur)
             w = np.where(cur > 0, cur * lap, 0)
             amp = w @ self.u_t
             pmf = amp.real ** 2 + amp.imag ** 2
@@ -218,6 +235,20 @@ class _Lockstep:
                 fail = live[over | short]
                 if fail.size:
                     self._restart(fail)
+
+
+def _minors_flops(rows: np.ndarray, cols: np.ndarray) -> float:
+    """Algorithmic FLOP of the column-minor sweeps of a round (the planner's
+    count, T (2 m_c + 6 (3 (m_c - 1) + 2 (n + 1 - m_c)) + 4 (m_c + 1)) per
+    problem, T = prod (rho_i + 1) after the min-row prepend)."""
+    occupied = rows > 0
+    n = rows.sum(axis=1)
+    big = np.iinfo(np.int32).max
+    rmin = np.where(occupied, rows, big).min(axis=1)
+    terms = np.prod(np.where(occupied, rows + 1.0, 1.0), axis=1) * rmin / (rmin + 1.0)
+    mc = (cols > 0).sum(axis=1)
+    per = 2.0 * mc + 6.0 * (3.0 * (mc - 1) + 2.0 * (n + 1 - mc)) + 4.0 * (mc + 1)
+    return float(np.sum(np.where(n > 0, terms * per, 0.0)))
 
 
 def _first_quantized(occupation) -> np.ndarray:
@@ -287,6 +318,28 @@ def generate_samples(input, shots: int, interferometer,
     minors = _MinorsBatch(u)
     if shots == 0:
         return []
+    t0 = time.perf_counter()
+    engines = []
+    out = _sample_shots(minors, occ, shots, seed, reject_condition, ps_modes, ps_photons,
+                        max_trials, uniform_particle_overlap, engines)
+    last_run.clear()
+    last_run.update(
+        shots=shots, photons=n, modes=d,
+        rounds=sum(e.rounds for e in engines),
+        problems=sum(e.problems for e in engines),
+        minors_flops=sum(e.flops for e in engines),
+        minors_seconds=minors.seconds,
+        seconds=time.perf_counter() - t0,
+    )
+    return out
+
+
+def _sample_shots(minors, occ, shots, seed, reject_condition, ps_modes, ps_photons, max_trials,
+                  uniform_particle_overlap, engines):
+    u = minors.u
+    d = len(occ)
+    n = int(occ.sum())
+    postselected = len(ps_modes) > 0
     rngs = [np.random.default_rng(seed=seed + s) for s in range(shots)]
 
     if postselected and reject_condition is not None:
@@ -297,6 +350,7 @@ def generate_samples(input, shots: int, interferometer,
             live_reject = lambda idx: np.array([bool(reject_condition()) for _ in idx])  # noqa: E731
             eng = _Lockstep(minors, [rngs[s]], [_first_quantized(occ)], live_reject,
                             (ps_modes, ps_photons, max_trials))
+            engines.append(eng)
             smp = eng.run()[0]
             out.append(tuple(int(v) for v in np.delete(smp, list(ps_modes))))
         return out
@@ -329,6 +383,7 @@ def generate_samples(input, shots: int, interferometer,
             pools.append(_first_quantized(indist))
     eng = _Lockstep(minors, rngs, pools, reject_many if decisions is not None else None,
                     (ps_modes, ps_photons, max_trials) if postselected else None)
+    engines.append(eng)
     smp = eng.run().astype(np.int64)
     if dist_out is not None:
         smp = dist_out + smp

@@ -14,6 +14,9 @@ Workloads (sizes from the engine's benchmark definition):
   batched_uneven    4096 submatrices of a 48-mode Haar unitary, n ~ U{8..24},
                     row / column multiplicities in {1, 2, 3} summing to n
   gradient_batch    256 Haar submatrices, n ~ U{12..20}, multiplicities 1
+  sampling          Clifford & Clifford sampling: 40-mode Haar unitary, one
+                    photon in each of modes 0..19 (minors_flops counts the
+                    column-minor work of one shot's samples)
 """
 
 from __future__ import annotations
@@ -27,6 +30,8 @@ __all__ = [
     "large_single",
     "batched_uneven",
     "gradient_batch",
+    "sampling",
+    "minors_flops",
     "glynn_terms",
     "glynn_flops",
 ]
@@ -152,3 +157,29 @@ def glynn_flops(rows, cols) -> float:
         return 0.0
     mc = int(np.count_nonzero(c))
     return float(glynn_terms(rows)) * (2.0 * mc + 6.0 * (n - 1) + 4.0)
+
+
+def sampling(seed: int = 6, modes: int = 40, photons: int = 20):
+    """Returns (U, input occupation) for boson sampling of `photons` single
+    photons in the first modes of a `modes`-mode Haar interferometer."""
+    rng = np.random.default_rng(seed)
+    u = haar_unitary(modes, rng)
+    occ = np.zeros(modes, dtype=np.int64)
+    occ[:photons] = 1
+    return u, occ
+
+
+def minors_flops(rows, cols) -> float:
+    """Algorithmic FLOP of one column-minor sweep perm(A; r, c - e_l) for all
+    l, the planner's count: T terms of (row update 2 m_c, prefix / suffix /
+    combine products 3 (m_c - 1) complex multiplies plus 2 (n + 1 - m_c) for
+    repeated columns, m_c + 1 weighted accumulations of 4), T from the
+    prepended row radices; 0 for an empty row set."""
+    rows = np.asarray(rows)
+    c = np.asarray(cols)
+    if rows.sum() == 0:
+        return 0.0
+    mc = int(np.count_nonzero(c))
+    n = int(rows.sum())
+    return float(glynn_terms(rows)) * (2.0 * mc + 6.0 * (3.0 * (mc - 1) + 2.0 * (n + 1 - mc))
+                                       + 4.0 * (mc + 1))

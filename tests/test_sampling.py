@@ -86,6 +86,7 @@ class _OracleMinors:
         self.O = O
         self.u = np.ascontiguousarray(u, dtype=np.complex128)
         self.d = self.u.shape[0]
+        self.seconds = 0.0
 
     def __call__(self, rows, cols):
         out = np.zeros(rows.shape, dtype=np.complex128)
