@@ -171,6 +171,34 @@ int pqp_laplace_batch_c128(int64_t count, const double *const *matrices, const i
                            const int32_t *const *cols, double *const *outs);
 
 /*
+ * Tensor permanent of n uniformly partially distinguishable bosons with
+ * particle overlap x, the quantity behind piquasso's detection probability
+ * of such inputs (_calculate_tensor_permanent,
+ * piquasso/_simulators/passive/probabilities.py:193-237, called from
+ * get_partially_distinguishable_detection_probability, :156-190):
+ *   T = sum over column subsets s, t of (-1)^(|s|+|t|)
+ *       prod_i ( x R_s[i] conj(R_t[i]) + (1 - x) Q_{s&t}[i] ),
+ *   R_s[i] = sum_{c in s} A[i, c],  Q_u[i] = sum_{c in u} |A[i, c]|^2,
+ * for the n x n complex matrix A (row-major interleaved, leading dimension
+ * ld >= n; the caller passes the multiplicity-expanded interferometer).
+ * The kernels sum the same value over pairs of sign vectors (Glynn's form,
+ * 4^(n-1) pairs, much less cancellation than the subsets).
+ * T is real (the reference returns it in a complex dtype with a zero
+ * imaginary part): *out = T, summed in double-double and rounded.  The
+ * alternating sum cancels heavily; *abs_sum (may be null) receives the sum
+ * of the absolute values of the summed terms, so that about
+ * 8 (n + 2) 2^-53 abs_sum bounds the error of T.  n = 0 gives 1; n <= 20.
+ */
+int pqp_tensor_permanent_c128(const double *matrix, int64_t n, int64_t ld, double overlap,
+                              double *out, double *abs_sum);
+
+/* The same for `count` matrices in one launch per matrix order: out[k] and
+ * abs_sum[k] (may be null). */
+int pqp_tensor_permanent_batch_c128(int64_t count, const double *const *matrices, const int64_t *n,
+                                    const int64_t *ld, const double *overlap, double *out,
+                                    double *abs_sum);
+
+/*
  * Gradients of `count` permanents: grads[k] (m_r[k] x m_c[k] complex,
  * row-major, interleaved) = d perm(A_k; r_k, c_k) / d (A_k)_ij
  * = r_i c_j perm(A_k; r_k - e_i, c_k - e_j) (0 where r_i or c_j is 0), the

@@ -73,6 +73,14 @@ int pqo_gray_states(const int *limits, int n_digits, int64_t offset,
 /* Number of processors OpenMP sees (used for the default chunking). */
 int pqo_num_procs(void);
 
+/* Tensor permanent of uniformly partially distinguishable bosons
+ * (piquasso/_simulators/passive/probabilities.py:193-237) of the n x n
+ * matrix a (interleaved complex, leading dimension n) at particle overlap
+ * x, in double-double: out = {re_hi, re_lo, im_hi, im_lo}; abs_sum (may be
+ * null) = sum over subset pairs of |term|.  n <= 16; n = 0 gives 1. */
+int pqo_tensor_perm_dd(const double *a, int n, double x, int n_jobs, double *out,
+                       double *abs_sum);
+
 #ifdef __cplusplus
 }
 #endif

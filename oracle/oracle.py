@@ -42,6 +42,7 @@ from __future__ import annotations
 import ctypes
 import importlib.util
 import itertools
+import math
 import os
 import sysconfig
 from fractions import Fraction
@@ -80,6 +81,7 @@ def lib():
             ("pqo_grad_dd_c128", [dp, ci, ci, ip, ip, ci, dp]),
             ("pqo_laplace_dd_c128", [dp, ci, ci, ip, ip, ci, dp]),
             ("pqo_gray_states", [ip, ci, ctypes.c_int64, ctypes.c_int64, ip]),
+            ("pqo_tensor_perm_dd", [dp, ci, ctypes.c_double, ci, dp, dp]),
         ):
             fn = getattr(L, name)
             fn.argtypes = argt
@@ -204,6 +206,86 @@ def ref_laplace(matrix, rows, cols) -> np.ndarray:
     )
     _check(min(size, 0))  # the shim returns the result length
     return out[:size]
+
+
+def tensor_perm_dd(matrix, overlap: float, n_jobs: int = 0):
+    """Tensor permanent of uniformly partially distinguishable bosons
+    (piquasso/_simulators/passive/probabilities.py:193-237) in double-double:
+    returns (value as complex, sum over subset pairs of |term|)."""
+    a = np.ascontiguousarray(np.asarray(matrix, dtype=np.complex128))
+    n = a.shape[0]
+    if a.ndim != 2 or a.shape[1] != n:
+        raise ValueError("tensor permanent needs a square matrix")
+    out = np.zeros(4)
+    asum = np.zeros(1)
+    _check(lib().pqo_tensor_perm_dd(_ptr(a.view(np.float64), ctypes.c_double), n, float(overlap),
+                                    n_jobs, _ptr(out, ctypes.c_double), _ptr(asum, ctypes.c_double)))
+    return complex(out[0] + out[1], out[2] + out[3]), float(asum[0])
+
+
+def tensor_perm_bruteforce(matrix, overlap, exact: bool = False):
+    """The tensor permanent by its permutation form: the sum over pairs of
+    permutations (sigma, tau) of prod_i A[i, sigma(i)] conj(A[i, tau(i)]),
+    each weighted by overlap^(number of rows where sigma(i) != tau(i)) — the
+    expansion of the subset formula (a row whose two column choices agree
+    gets x + (1 - x) = 1).  With ``exact=True`` the entries and the overlap
+    become Fractions and the exact value (re, im) is returned."""
+    a = np.asarray(matrix)
+    n = a.shape[0]
+    if n == 0:
+        return (Fraction(1), Fraction(0)) if exact else 1.0 + 0j
+    if exact:
+        fr = [[(Fraction(float(np.real(a[i, j]))), Fraction(float(np.imag(a[i, j]))))
+               for j in range(n)] for i in range(n)]
+        x = Fraction(overlap)
+    perms = list(itertools.permutations(range(n)))
+    tr, ti = (Fraction(0), Fraction(0)) if exact else (0.0, 0.0)
+    total = 0j
+    for p in perms:
+        for q in perms:
+            k = sum(1 for i in range(n) if p[i] != q[i])
+            if exact:
+                pr, pi = x ** k, Fraction(0)
+                for i in range(n):
+                    ar, ai = fr[i][p[i]]
+                    br, bi = fr[i][q[i]]
+                    # a * conj(b)
+                    cr, ci_ = ar * br + ai * bi, ai * br - ar * bi
+                    pr, pi = pr * cr - pi * ci_, pr * ci_ + pi * cr
+                tr += pr
+                ti += pi
+            else:
+                prod = overlap ** k + 0j
+                for i in range(n):
+                    prod *= a[i, p[i]] * np.conj(a[i, q[i]])
+                total += prod
+    return (tr, ti) if exact else total
+
+
+def uniform_input_norm(input_occupation, overlap: float) -> float:
+    """prod_j sum_k C(n_j, k) x^k (1 - x)^(n_j - k) k!
+    (piquasso/_simulators/passive/probabilities.py:387-398)."""
+    norm = 1.0
+    for nj in input_occupation:
+        nj = int(nj)
+        norm *= sum(math.comb(nj, k) * overlap ** k * (1.0 - overlap) ** (nj - k) * math.factorial(k)
+                    for k in range(nj + 1))
+    return norm
+
+
+def pd_detection_probability(occupation, interferometer, input_occupation, overlap: float) -> float:
+    """Detection probability of an output occupation for uniformly partially
+    distinguishable input photons (probabilities.py:156-190): the tensor
+    permanent of the multiplicity-expanded interferometer over the input
+    norm times prod of output factorials; 0 when the photon numbers differ."""
+    occ = np.asarray(occupation, dtype=int)
+    inp = np.asarray(input_occupation, dtype=int)
+    if occ.sum() != inp.sum():
+        return 0.0
+    a = assym_reduce(np.asarray(interferometer), occ, inp)
+    t, _ = tensor_perm_dd(a, overlap)
+    den = uniform_input_norm(inp, overlap) * float(np.prod([math.factorial(int(k)) for k in occ]))
+    return t.real / den
 
 
 def gray_states(limits, offset: int = 0, count: Optional[int] = None) -> np.ndarray:

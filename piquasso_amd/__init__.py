@@ -13,18 +13,22 @@ torch_permanent(A, rows, cols)       differentiable permanent (torch.autograd.Fu
 output_probabilities(U, s, outputs)  boson-sampling probabilities of many outputs in one batch
 permanent_laplace_batch(ms, rs, cs)  many permanent_laplace problems in one launch
 generate_samples / generate_lossy_samples  Clifford & Clifford sampling, shots in lockstep
+tensor_permanent / detection_probabilities  uniformly partially distinguishable photons
 """
 
 from ._math.permanent import permanent, permanent_laplace, permanent_laplace_batch
 from .engine import PermanentBatch, permanent_batch, permanent_ex
 from .grad import PermanentFunction, permanent_grad, permanent_grad_batch, torch_permanent
 from .probabilities import output_amplitudes, output_probabilities
+from .distinguishability import (detection_probabilities, detection_probability, tensor_permanent,
+                                 tensor_permanent_batch)
 from .sampling import generate_lossy_samples, generate_samples
 
 __all__ = [
     "permanent", "permanent_laplace", "permanent_ex", "PermanentBatch", "permanent_batch",
     "permanent_grad", "permanent_grad_batch", "PermanentFunction", "torch_permanent",
     "output_amplitudes", "output_probabilities", "permanent_laplace_batch",
-    "generate_samples", "generate_lossy_samples",
+    "generate_samples", "generate_lossy_samples", "tensor_permanent", "tensor_permanent_batch",
+    "detection_probability", "detection_probabilities",
 ]
 __version__ = "0.1.0"

@@ -79,6 +79,13 @@ SIGNATURES = {
         [_c.c_int64, _c.POINTER(_dp), _i64p, _i64p, _i64p, _c.POINTER(_i32p),
          _c.POINTER(_i32p), _c.POINTER(_dp)],
     ),
+    "pqp_tensor_permanent_c128": (
+        _c.c_int, [_dp, _c.c_int64, _c.c_int64, _c.c_double, _dp, _dp],
+    ),
+    "pqp_tensor_permanent_batch_c128": (
+        _c.c_int,
+        [_c.c_int64, _c.POINTER(_dp), _i64p, _i64p, _dp, _dp, _dp],
+    ),
     "pqp_grad_batch_c128": (
         _c.c_int,
         [_c.c_int64, _c.POINTER(_dp), _i64p, _i64p, _i64p, _c.POINTER(_i32p),

@@ -463,6 +463,128 @@ void scatter_gradients(const Plan &mp, const std::vector<double> &res,
 }  // namespace
 
 namespace {
+
+// ------------------------------------------------------- tensor permanents --
+
+// Host image of a batch of tensor permanents: items, work units (grouped by
+// matrix order, one launch per order) and per-item tables, laid out as one
+// arena (tensor_types.hpp).
+struct TensorPlan {
+  std::vector<char> arena;
+  int64_t items_off = 0, units_off = 0;
+  std::vector<TensorItem> items;
+  std::vector<TensorUnit> units;
+  std::vector<std::pair<int, std::pair<int, int>>> launches;  // (n, (first unit, count))
+};
+
+// Steps per unit 2^kt for sign vectors of nb free signs: about 2^20 units
+// at most (large n) and at least ~4096 where the pair count allows (small
+// n), so the grid fills the chip.
+int tensor_walk_bits(int nb, int ls) {
+  auto units = [&](int kt) {
+    return std::ldexp(1.0, (nb - ls) + (nb - kt)) / 2.0;
+  };
+  int kt = std::min(nb, 10);
+  while (kt < nb && units(kt) > 1048576.0) ++kt;
+  while (kt > 1 && units(kt) < 4096.0) --kt;
+  return kt;
+}
+
+int tensor_plan(int64_t count, const double *const *matrices, const int64_t *n, const int64_t *ld,
+                const double *overlap, TensorPlan &tp, std::string &err) {
+  tp = TensorPlan();
+  std::vector<int64_t> order(static_cast<size_t>(count));
+  for (int64_t k = 0; k < count; ++k) {
+    const int64_t nk = n[k];
+    if (nk < 0 || nk > kTensorMaxN) {
+      err = "item " + std::to_string(k) + ": tensor permanents support matrices of order 0.." +
+            std::to_string(kTensorMaxN);
+      return nk < 0 ? PQP_ERR_INVALID : PQP_ERR_RANGE;
+    }
+    if (nk > 0 && (!matrices[k] || ld[k] < nk)) {
+      err = "item " + std::to_string(k) + ": invalid matrix pointer or leading dimension";
+      return PQP_ERR_INVALID;
+    }
+    if (!std::isfinite(overlap[k])) {
+      err = "item " + std::to_string(k) + ": particle overlap is not finite";
+      return PQP_ERR_INVALID;
+    }
+    order[k] = k;
+  }
+  // units grouped by matrix order (one launch each)
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return n[a] < n[b]; });
+  tp.items.resize(static_cast<size_t>(count));
+  for (int64_t k : order) {
+    TensorItem &it = tp.items[k];
+    std::memset(&it, 0, sizeof(it));
+    const int nk = static_cast<int>(n[k]);
+    it.n = nk;
+    it.x = overlap[k];
+    it.unit0 = static_cast<int32_t>(tp.units.size());
+    if (nk == 0) continue;
+    const int nb = nk - 1;  // free signs of a sign vector
+    it.ls = std::min(nb, 6);
+    it.kt = tensor_walk_bits(nb, it.ls);
+    const uint32_t n_s = 1u << (nb - it.ls);
+    const uint32_t n_t = 1u << (nb - it.kt);
+    for (uint32_t sh = 0; sh < n_s; ++sh)
+      for (uint32_t tb = 0; tb < n_t; ++tb) {
+        const uint64_t lo = static_cast<uint64_t>(tb) << it.kt;
+        const uint64_t hi = lo + (1ull << it.kt) - 1;
+        if ((hi >> it.ls) < sh) continue;  // every pair below the diagonal block
+        TensorUnit u;
+        u.item = static_cast<int32_t>(k);
+        u.s_hi = static_cast<int32_t>(sh);
+        u.t_blk = static_cast<int32_t>(tb);
+        u.mixed = (lo >> it.ls) <= sh ? 1 : 0;
+        tp.units.push_back(u);
+      }
+    it.n_units = static_cast<int32_t>(tp.units.size()) - it.unit0;
+    if (tp.units.size() > static_cast<size_t>(INT32_MAX)) {
+      err = "too many work units";
+      return PQP_ERR_RANGE;
+    }
+    if (!tp.launches.empty() && tp.launches.back().first == nk)
+      tp.launches.back().second.second += it.n_units;
+    else
+      tp.launches.push_back({nk, {it.unit0, it.n_units}});
+  }
+  // arena: items | units | per item: columns (complex, [c][i]), (1 - x)|a|^2 ([c][i])
+  auto align = [](int64_t v) { return (v + 255) & ~int64_t(255); };
+  int64_t off = 0;
+  tp.items_off = off;
+  off = align(off + static_cast<int64_t>(tp.items.size() * sizeof(TensorItem)));
+  tp.units_off = off;
+  off = align(off + static_cast<int64_t>(tp.units.size() * sizeof(TensorUnit)));
+  for (int64_t k = 0; k < count; ++k) {
+    TensorItem &it = tp.items[k];
+    const int64_t nn = static_cast<int64_t>(it.n) * it.n;
+    it.cols_off = off;
+    off = align(off + nn * 16);
+    it.pw_off = off;
+    off = align(off + nn * 8);
+  }
+  tp.arena.assign(static_cast<size_t>(std::max<int64_t>(off, 256)), 0);
+  std::memcpy(tp.arena.data() + tp.items_off, tp.items.data(), tp.items.size() * sizeof(TensorItem));
+  std::memcpy(tp.arena.data() + tp.units_off, tp.units.data(), tp.units.size() * sizeof(TensorUnit));
+  for (int64_t k = 0; k < count; ++k) {
+    const TensorItem &it = tp.items[k];
+    const int nk = it.n;
+    double *colp = reinterpret_cast<double *>(tp.arena.data() + it.cols_off);
+    double *pwp = reinterpret_cast<double *>(tp.arena.data() + it.pw_off);
+    const double y = 1.0 - it.x;
+    for (int c = 0; c < nk; ++c)
+      for (int i = 0; i < nk; ++i) {
+        const double re = matrices[k][2 * (i * ld[k] + c)];
+        const double im = matrices[k][2 * (i * ld[k] + c) + 1];
+        colp[2 * (c * nk + i)] = re;
+        colp[2 * (c * nk + i) + 1] = im;
+        pwp[c * nk + i] = y * (re * re + im * im);
+      }
+  }
+  return PQP_OK;
+}
+
 // One evaluation of a batch: the permanents, then (gradient batches) the
 // column minors on the same stream.
 int enqueue_all(pqp_batch *b, int mode, hipEvent_t ev_begin, hipEvent_t ev_end) {
@@ -609,6 +731,64 @@ int pqp_laplace_batch_c128(int64_t count, const double *const *matrices, const i
     }
   }
   return PQP_OK;
+}
+
+int pqp_tensor_permanent_batch_c128(int64_t count, const double *const *matrices, const int64_t *n,
+                                    const int64_t *ld, const double *overlap, double *out,
+                                    double *abs_sum) {
+  if (count < 0 || (count > 0 && (!matrices || !n || !ld || !overlap || !out)))
+    return fail(PQP_ERR_INVALID, "invalid tensor permanent arguments");
+  if (count > INT32_MAX) return fail(PQP_ERR_RANGE, "too many items");
+  if (count == 0) return PQP_OK;
+  TensorPlan tp;
+  std::string err;
+  int st = tensor_plan(count, matrices, n, ld, overlap, tp, err);
+  if (st != PQP_OK) return fail(st, err);
+  if ((st = require_device()) != PQP_OK) return st;
+  SingleState &S = single_state();
+  std::lock_guard<std::mutex> lock(S.mu);
+  Exec &x = S.x;
+  if (!S.init) {
+    PQP_HIP(hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking));
+    x.own_stream = true;
+    S.init = true;
+  }
+  if ((st = x.arena.ensure(tp.arena.size())) != PQP_OK) return st;
+  if ((st = x.partials.ensure(std::max<size_t>(tp.units.size(), 1) * 3 * sizeof(double))) != PQP_OK)
+    return st;
+  if ((st = x.results.ensure(static_cast<size_t>(count) * 2 * sizeof(double))) != PQP_OK) return st;
+  if ((st = x.stage.ensure(std::max(tp.arena.size(), static_cast<size_t>(count) * 2 * sizeof(double)))) !=
+      PQP_OK)
+    return st;
+  // the plan arena of the single-call state is replaced: force its re-upload
+  x.plan = Plan();
+  std::memcpy(x.stage.p, tp.arena.data(), tp.arena.size());
+  PQP_HIP(hipMemcpyAsync(x.arena.p, x.stage.p, tp.arena.size(), hipMemcpyHostToDevice, x.stream));
+  TensorArgs a;
+  a.arena = static_cast<const char *>(x.arena.p);
+  a.items = reinterpret_cast<const TensorItem *>(static_cast<const char *>(x.arena.p) + tp.items_off);
+  a.units = reinterpret_cast<const TensorUnit *>(static_cast<const char *>(x.arena.p) + tp.units_off);
+  a.partials = static_cast<double *>(x.partials.p);
+  a.out = static_cast<double *>(x.results.p);
+  a.n_units = static_cast<int32_t>(tp.units.size());
+  a.n_items = static_cast<int32_t>(count);
+  for (const auto &l : tp.launches) PQP_HIP(launch_tensor(l.first, a, l.second.first, l.second.second, x.stream));
+  PQP_HIP(launch_tensor_finalize(a, x.stream));
+  PQP_HIP(hipMemcpyAsync(x.stage.p, x.results.p, static_cast<size_t>(count) * 2 * sizeof(double),
+                         hipMemcpyDeviceToHost, x.stream));
+  PQP_HIP(hipStreamSynchronize(x.stream));
+  const double *r = static_cast<const double *>(x.stage.p);
+  for (int64_t k = 0; k < count; ++k) {
+    out[k] = r[2 * k];
+    if (abs_sum) abs_sum[k] = r[2 * k + 1];
+  }
+  return PQP_OK;
+}
+
+int pqp_tensor_permanent_c128(const double *matrix, int64_t n, int64_t ld, double overlap,
+                              double *out, double *abs_sum) {
+  const double *m[1] = {matrix};
+  return pqp_tensor_permanent_batch_c128(1, m, &n, &ld, &overlap, out, abs_sum);
 }
 
 int pqp_grad_batch_c128(int64_t count, const double *const *matrices, const int64_t *m_r,

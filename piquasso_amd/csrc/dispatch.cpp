@@ -64,4 +64,10 @@ hipError_t launch_minors(int bucket, const LaunchArgs &a, int first_unit, int n_
   }
 }
 
+hipError_t launch_tensor(int n, const TensorArgs &a, int first_unit, int n_units, hipStream_t s) {
+  if (n >= 1 && n <= 12) return launch_tensor_p0(n, a, first_unit, n_units, s);
+  if (n >= 13 && n <= kTensorMaxN) return launch_tensor_p1(n, a, first_unit, n_units, s);
+  return hipErrorInvalidValue;
+}
+
 }  // namespace pqp

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include "pqp_types.hpp"
+#include "tensor_types.hpp"
 
 namespace pqp {
 
@@ -56,6 +57,13 @@ hipError_t launch_minors_b8(const LaunchArgs &a, int first_unit, int n_units, hi
 hipError_t launch_minors(int bucket, const LaunchArgs &a, int first_unit, int n_units,
                          hipStream_t stream);
 hipError_t launch_minors_finalize(const LaunchArgs &a, int n_items, hipStream_t stream);
+
+// Tensor permanents (tensor_perm.hip): one launch per matrix order n over
+// units [first_unit, first_unit + n_units), then one finalize for all items.
+hipError_t launch_tensor_p0(int n, const TensorArgs &a, int first_unit, int n_units, hipStream_t s);
+hipError_t launch_tensor_p1(int n, const TensorArgs &a, int first_unit, int n_units, hipStream_t s);
+hipError_t launch_tensor(int n, const TensorArgs &a, int first_unit, int n_units, hipStream_t s);
+hipError_t launch_tensor_finalize(const TensorArgs &a, hipStream_t stream);
 
 hipError_t launch_fwd(int bucket, const FwdLaunch &l);
 hipError_t occupancy_fwd(int bucket, bool persistent, bool fp32, bool comp, int *blocks_per_cu);
