@@ -102,21 +102,21 @@ __global__ __launch_bounds__(kWave) void tensor_perm_kernel(TensorArgs a, int fi
   double acc = 0.0;
   double aabs = 0.0;  // sum of |weight x term| (forward-error scale)
   const int steps = 1 << kt;
+  // The step after state k flips eps at column c = ctz(k + 1) + 1:
+  // R_eps -+= 2 A[:, c], Q -+= 2 delta_c (1 - x)|A[:, c]|^2.  A step's
+  // column is 3n doubles of scalar operands, more than the SGPR file holds
+  // at once for large n: the first PF rows of A's column are loaded before
+  // the product of state k (their latency hides behind it), the rest after.
+  constexpr int PF = N < 8 ? N : 8;
   for (int k = 0; k < steps; ++k) {
-    if (k) {
-      // eps flips at column c = b + 1: R += -+2 A[:, c], Q += -+2 delta_c (1 - x)|A[:, c]|^2
-      const int b = __builtin_ctz(static_cast<unsigned>(k));
-      const int c = b + 1;
-      const uint32_t g = static_cast<uint32_t>(k ^ (k >> 1));
-      const double sd2 = in_vgpr(((g >> b) & 1u) ? -2.0 : 2.0);
-      const double sq2 = ((s >> b) & 1u) ? -sd2 : sd2;
+    const bool more = k + 1 < steps;  // wave-uniform
+    const int b = __builtin_ctz(static_cast<unsigned>(k + 1));
+    const int c = more ? b + 1 : 0;
+    double ur[PF], ui[PF];
 #pragma unroll
-      for (int i = 0; i < N; ++i) {
-        rr[i] = fmaR(sd2, col[2 * (c * N + i)], rr[i]);
-        ri[i] = fmaR(sd2, col[2 * (c * N + i) + 1], ri[i]);
-        q[i] = fmaR(sq2, pw[c * N + i], q[i]);
-      }
-      par_t ^= 1;
+    for (int i = 0; i < PF; ++i) {
+      ur[i] = col[2 * (c * N + i)];
+      ui[i] = col[2 * (c * N + i) + 1];
     }
     double w = 2.0;
     if (mixed) {
@@ -145,6 +145,24 @@ __global__ __launch_bounds__(kWave) void tensor_perm_kernel(TensorArgs a, int fi
     if ((k & (kDdFold - 1)) == kDdFold - 1) {
       acc_dd = dd_add_d(acc_dd, acc);
       acc = 0.0;
+    }
+    if (more) {
+      const uint32_t g = static_cast<uint32_t>((k + 1) ^ ((k + 1) >> 1));
+      const double sd2 = in_vgpr(((g >> b) & 1u) ? -2.0 : 2.0);
+      const double sq2 = ((s >> b) & 1u) ? -sd2 : sd2;
+#pragma unroll
+      for (int i = 0; i < PF; ++i) {
+        rr[i] = fmaR(sd2, ur[i], rr[i]);
+        ri[i] = fmaR(sd2, ui[i], ri[i]);
+      }
+#pragma unroll
+      for (int i = PF; i < N; ++i) {
+        rr[i] = fmaR(sd2, col[2 * (c * N + i)], rr[i]);
+        ri[i] = fmaR(sd2, col[2 * (c * N + i) + 1], ri[i]);
+      }
+#pragma unroll
+      for (int i = 0; i < N; ++i) q[i] = fmaR(sq2, pw[c * N + i], q[i]);
+      par_t ^= 1;
     }
   }
   acc_dd = dd_add_d(acc_dd, acc);
