@@ -13,6 +13,9 @@ Printed as ONE JSON line (rank 0):
                   permanent (speedup_vs_plain = value / plain rate)
   n32             one n = 32 permanent (2^31 Gray terms): time, % of FP64 peak
   gradient        gradients/s of 256 permanents n ~ U{12..20} (d perm / dA)
+  tensor          tensor permanent of 18 partially distinguishable photons
+                  (detection probabilities): time and % of FP64 peak; the CPU
+                  figure is a numpy restatement of the reference's loop at n = 9
   sampling        Clifford & Clifford boson samples/s (40 modes, 20 photons,
                   1024 shots in lockstep: one batched column-minor launch per
                   photon round), checked shot for shot against the reference's
@@ -48,6 +51,8 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 SAMPLING_SHOTS = 1024
 SAMPLING_SEED = 1234
 SAMPLING_CPU_SECONDS = 8.0
+TENSOR_N = 18
+TENSOR_CPU_N = 9
 
 
 def _load_batched():
@@ -126,7 +131,20 @@ def cpu_baseline_child(seconds: float) -> None:
         singles[cid] = _median_call_us(lambda: fn(a, r, c), reps)
     print(json.dumps({"kind": kind, "items": done, "seconds": dt, "flops": flops,
                       "threads": int(os.environ.get("OMP_THREAD_LIMIT", "0") or 0),
-                      "singles_us": singles, "sampling": _cpu_sampling(SAMPLING_CPU_SECONDS)}))
+                      "singles_us": singles, "sampling": _cpu_sampling(SAMPLING_CPU_SECONDS),
+                      "tensor": _cpu_tensor()}))
+
+
+def _cpu_tensor():
+    """The reference's tensor-permanent loop (numpy restatement) at n = 9."""
+    from oracle.tensor_port import tensor_perm_port
+    from piquasso_amd import workloads as W
+
+    a = W.tensor(TENSOR_CPU_N)
+    t0 = time.perf_counter()
+    tensor_perm_port(a, 0.5)
+    dt = time.perf_counter() - t0
+    return {"n": TENSOR_CPU_N, "seconds": dt, "pairs": 4.0 ** TENSOR_CPU_N}
 
 
 def _cpu_sampling(seconds: float):
@@ -181,6 +199,7 @@ def run_cpu_baseline(seconds: float):
         "gflops": rec["flops"] / rec["seconds"] / 1e9 if rec["seconds"] > 0 else None,
         "singles_us": rec.get("singles_us"),
         "sampling": rec.get("sampling"),
+        "tensor": rec.get("tensor"),
     }
 
 
@@ -226,6 +245,40 @@ def _gradient_leg(pa, W, reps: int):
         "what": "256 gradients d perm/dA (n~U{12..20}, 48-mode Haar submatrices): permanent "
                 "sweep + one column-minor sweep per row, one hipGraph replay",
     }
+
+
+def _tensor_leg(pa, W, cpu):
+    """One tensor permanent of n = 18 (x = 0.5): 4^17 / 2 sign-vector pairs,
+    timed around the whole call (planning and transfers are microseconds)."""
+    a = W.tensor(TENSOR_N)
+    pa.tensor_permanent(a, 0.5)
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        (v,), (asum,) = pa.tensor_permanent_batch([a], [0.5], with_abs_sum=True)
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    fl = W.tensor_flops(TENSOR_N)
+    rec = {
+        "n": TENSOR_N,
+        "ms": t * 1e3,
+        "tflops": fl / t / 1e12,
+        "frac_fp64_peak": fl / t / 1e12 / FP64_PEAK_TFLOPS,
+        "value": v,
+        "error_bound": 8 * (TENSOR_N + 2) * np.finfo(float).eps * asum,
+        "what": "tensor permanent (uniform overlap x = 0.5) of an 18 x 18 complex Gaussian matrix, "
+                "sign-vector form, 4^17 / 2 pairs",
+    }
+    ct = (cpu or {}).get("tensor")
+    if ct:
+        rec["reference_cpu"] = {
+            "pairs_per_s": ct["pairs"] / ct["seconds"],
+            "kind": "port",
+            "sample": f"numpy restatement of the reference's subset-pair loop, n = {ct['n']} "
+                      f"({ct['pairs']:.0f} pairs) in {ct['seconds']:.2f} s, one thread",
+        }
+        rec["gpu_pairs_per_s"] = 4.0 ** (TENSOR_N - 1) / 2 / t
+    return rec
 
 
 def _sampling_leg(pa, W, cpu):
@@ -276,7 +329,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-child", type=float, default=None)
-    ap.add_argument("--profile", choices=["batch", "n32", "grad", "sampling"], default=None,
+    ap.add_argument("--profile", choices=["batch", "n32", "grad", "sampling", "tensor"], default=None,
                     help="profiling runs: only launch this workload's kernels --steps times")
     ap.add_argument("--replay", choices=["eager", "graph"], default="eager",
                     help="profiling runs: direct launches or hipGraph replays")
@@ -302,6 +355,14 @@ def main():
     if args.profile is not None:
         import piquasso_amd as pa
 
+        if args.profile == "tensor":
+            from piquasso_amd import workloads as W
+
+            a = W.tensor(TENSOR_N)
+            for _ in range(args.steps):
+                pa.tensor_permanent(a, 0.5)
+            print(json.dumps({"profile": "tensor", "n": TENSOR_N}))
+            return 0
         if args.profile == "sampling":
             from piquasso_amd import workloads as W
 
@@ -387,6 +448,7 @@ def main():
 
     grad = _gradient_leg(pa, W, reps=max(3, min(args.steps, 10)))
     sampling = _sampling_leg(pa, W, cpu)
+    tensor = _tensor_leg(pa, W, cpu)
 
     # single permanents through the drop-in entry (host buffers in, result
     # out: planning, upload, sweep, download); the reference's CPU time for
@@ -445,6 +507,7 @@ def main():
         "n32": n32,
         "gradient": grad,
         "sampling": sampling,
+        "tensor_permanent": tensor,
         "single_permanents": singles,
         "roofline": {
             "bound": "mfma",
@@ -460,7 +523,7 @@ def main():
             "flop_model": "sum over items of T * (2 m_c + 6 (n - 1) + 4), T = Gray terms",
             "units": stats["units"],
         },
-        "cpu_baseline": {k: v for k, v in cpu.items() if k != "sampling"} if cpu else cpu,
+        "cpu_baseline": {k: v for k, v in cpu.items() if k not in ("sampling", "tensor")} if cpu else cpu,
     }
     print(json.dumps(line))
     return 0

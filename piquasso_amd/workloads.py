@@ -14,6 +14,8 @@ Workloads (sizes from the engine's benchmark definition):
   batched_uneven    4096 submatrices of a 48-mode Haar unitary, n ~ U{8..24},
                     row / column multiplicities in {1, 2, 3} summing to n
   gradient_batch    256 Haar submatrices, n ~ U{12..20}, multiplicities 1
+  tensor            n x n i.i.d. (N + iN) / sqrt(2n) matrix for the tensor
+                    permanent of partially distinguishable photons
   sampling          Clifford & Clifford sampling: 40-mode Haar unitary, one
                     photon in each of modes 0..19 (minors_flops counts the
                     column-minor work of one shot's samples)
@@ -32,6 +34,8 @@ __all__ = [
     "gradient_batch",
     "sampling",
     "minors_flops",
+    "tensor",
+    "tensor_flops",
     "glynn_terms",
     "glynn_flops",
 ]
@@ -183,3 +187,20 @@ def minors_flops(rows, cols) -> float:
     n = int(rows.sum())
     return float(glynn_terms(rows)) * (2.0 * mc + 6.0 * (3.0 * (mc - 1) + 2.0 * (n + 1 - mc))
                                        + 4.0 * (mc + 1))
+
+
+def tensor(n: int = 18, seed: int = 7) -> np.ndarray:
+    """An n x n complex Gaussian matrix (N + iN) / sqrt(2n)."""
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(2.0 * n)
+
+
+def tensor_flops(n: int) -> float:
+    """Algorithmic FLOP of one tensor permanent in the sign-vector form:
+    4^(n-1) / 2 pairs (the conjugate half is not visited) of n factors
+    x R conj(R') + Q (complex multiply 6, add 1), n - 1 complex products (6),
+    the Gray updates of R' (2 per row) and Q (1 per row) and the weighted
+    accumulation (2): 16 n - 4 per pair."""
+    if n <= 0:
+        return 0.0
+    return 4.0 ** (n - 1) / 2.0 * (16.0 * n - 4.0)

@@ -77,3 +77,16 @@ def test_uniform_input_norm_matches_oracle():
     # x = 1: prod n_j!, x = 0: 1
     assert uniform_input_norm([2, 3], 1.0) == math.factorial(2) * math.factorial(3)
     assert uniform_input_norm([2, 3], 0.0) == 1.0
+
+
+@pytest.mark.parametrize("n", [1, 3, 5])
+def test_port_matches_oracle(oracle, n):
+    """The plain-numpy restatement of the reference's loop (the CPU
+    baseline) computes the same value as the double-double oracle."""
+    from oracle.tensor_port import tensor_perm_port
+
+    a = _rand(n, 50 + n)
+    for x in (0.0, 0.6):
+        ref, asum = oracle.tensor_perm_dd(a, x)
+        got = tensor_perm_port(a, x)
+        assert abs(got - ref) <= 16 * (n + 2) * np.finfo(float).eps * asum
