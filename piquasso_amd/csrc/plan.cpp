@@ -108,6 +108,8 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
 
   plan.items.resize(n_items);
   plan.info.resize(n_items);
+  // typical item tables are a few KB: reserve once instead of regrowing
+  plan.arena.reserve(static_cast<size_t>(n_items) * 8192 + 65536);
   plan.items_off = arena_alloc(plan.arena, n_items * int64_t(sizeof(ItemDesc)));
 
   std::map<std::vector<int>, InnerTables> inner_cache;

@@ -51,9 +51,12 @@ __all__ = ["generate_samples", "generate_lossy_samples", "expanded_lossy_matrix"
            "SampleGenerationError", "last_run"]
 
 # Counters of the most recent generate_samples call in this process: rounds,
-# column-minor problems, their algorithmic FLOP, and the seconds spent in the
-# batched minors call (planning, upload, kernels, download) and in total.
+# column-minor problems, the seconds spent in the batched minors calls
+# (planning, upload, kernels, download) and in total, and, when
+# ``count_flops`` is set, the minors' algorithmic FLOP (off by default: it
+# costs host time every round).
 last_run: dict = {}
+count_flops = False
 
 
 class SampleGenerationError(RuntimeError):
@@ -92,10 +95,17 @@ class _MinorsBatch:
         finally:
             self.seconds += time.perf_counter() - t0
 
+    # problems per pqp_laplace_batch_c128 call: bounds the host plan and the
+    # device arena (~10 KB per 20-photon problem) whatever the shot count
+    max_problems = 8192
+
     def _run(self, rows: np.ndarray, cols: np.ndarray) -> np.ndarray:
         rows = np.ascontiguousarray(rows, dtype=np.int32)
         cols = np.ascontiguousarray(cols, dtype=np.int32)
         count, d = rows.shape
+        if count > self.max_problems:
+            return np.concatenate([self._run(rows[k:k + self.max_problems], cols[k:k + self.max_problems])
+                                   for k in range(0, count, self.max_problems)])
         out = np.zeros((count, 2 * d), dtype=np.float64)
         if count == 0:
             return out.view(np.complex128)
@@ -132,6 +142,8 @@ class _Lockstep:
         d = minors.d
         S = len(pools)
         self.rngs = rngs
+        self._ints = [r.integers for r in rngs]
+        self._rand = [r.random for r in rngs]
         self.n = np.array([len(p) for p in pools], dtype=np.int64)
         width = max(int(self.n.max()) if S else 0, 1)
         self.pool0 = np.full((S, width), -1, dtype=np.int64)
@@ -194,17 +206,23 @@ class _Lockstep:
             if live.size == 0:
                 return self.smp
             # grow the input by a uniformly chosen pool particle (sampling.py:197-205)
-            for s in live:
-                i = int(self.rngs[s].integers(0, self.plen[s], dtype=np.int64))
-                L = int(self.plen[s])
-                self.cur[s, self.pool[s, i]] += 1
-                self.pool[s, i: L - 1] = self.pool[s, i + 1: L]
-                self.plen[s] = L - 1
+            plen = self.plen[live]
+            ints = self._ints
+            pick = np.fromiter((ints[s](0, L, dtype=np.int64) for s, L in zip(live.tolist(), plen.tolist())),
+                               dtype=np.int64, count=live.size)
+            self.cur[live, self.pool[live, pick]] += 1  # one entry per shot: no repeated index
+            # drop the picked particle, keeping the pool's order
+            width = self.pool.shape[1]
+            j = np.arange(width)[None, :]
+            src = np.minimum(j + (j >= pick[:, None]), width - 1)
+            self.pool[live] = np.take_along_axis(self.pool[live], src, axis=1)
+            self.plen[live] = plen - 1
             # distribution of the next output mode (sampling.py:711-749)
             cur = self.cur[live]
             rows = self.smp[live]
             lap = self.minors(rows, cur)
-            self.flops += _minors_flops(rows, cur)
+            if count_flops:
+                self.flops += _minors_flops(rows, cur)
             w = np.where(cur > 0, cur * lap, 0)
             amp = w @ self.u_t
             pmf = amp.real ** 2 + amp.imag ** 2
@@ -213,7 +231,8 @@ class _Lockstep:
                 raise ValueError("probabilities contain NaN")
             cdf = np.cumsum(pmf / norm, axis=1)
             cdf /= cdf[:, -1:]
-            u = np.array([self.rngs[s].random() for s in live])
+            rand = self._rand
+            u = np.fromiter((rand[s]() for s in live.tolist()), dtype=np.float64, count=live.size)
             mode = (cdf <= u[:, None]).sum(axis=1)
             mode = np.minimum(mode, cdf.shape[1] - 1)
             self.smp[live, mode] += 1
@@ -327,7 +346,7 @@ def generate_samples(input, shots: int, interferometer,
         shots=shots, photons=n, modes=d,
         rounds=sum(e.rounds for e in engines),
         problems=sum(e.problems for e in engines),
-        minors_flops=sum(e.flops for e in engines),
+        minors_flops=sum(e.flops for e in engines) if count_flops else None,
         minors_seconds=minors.seconds,
         seconds=time.perf_counter() - t0,
     )
@@ -389,7 +408,7 @@ def _sample_shots(minors, occ, shots, seed, reject_condition, ps_modes, ps_photo
         smp = dist_out + smp
     if postselected:
         smp = np.delete(smp, list(ps_modes), axis=1)
-    return [tuple(int(v) for v in row) for row in smp]
+    return [tuple(row) for row in smp.tolist()]
 
 
 def expanded_lossy_matrix(interferometer) -> np.ndarray:

@@ -142,3 +142,17 @@ def test_sampler_larger_case_matches_oracle(gpu):
     got = pa.generate_samples(occ, 64, u, seed_sequence=77)
     want = S.generate_samples(occ, 64, u, 77)
     assert got == want
+
+
+def test_sampler_chunked_minors_calls(gpu, monkeypatch):
+    """A round's minor problems split over several batched calls give the
+    same samples (the problems are independent)."""
+    import piquasso_amd as pa
+    import piquasso_amd.sampling as P
+
+    u = _haar(8, 41)
+    occ = [1, 2, 0, 1, 1, 0, 0, 1]
+    want = pa.generate_samples(occ, 40, u, seed_sequence=5)
+    monkeypatch.setattr(P._MinorsBatch, "max_problems", 7)
+    got = pa.generate_samples(occ, 40, u, seed_sequence=5)
+    assert got == want
