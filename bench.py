@@ -53,6 +53,7 @@ SAMPLING_SHOTS = 1024
 SAMPLING_SEED = 1234
 SAMPLING_CPU_SECONDS = 8.0
 TENSOR_N = 18
+GRADIENT_CPU_SECONDS = 5.0
 TENSOR_CPU_N = 9
 
 
@@ -133,7 +134,48 @@ def cpu_baseline_child(seconds: float) -> None:
     print(json.dumps({"kind": kind, "items": done, "seconds": dt, "flops": flops,
                       "threads": int(os.environ.get("OMP_THREAD_LIMIT", "0") or 0),
                       "singles_us": singles, "sampling": _cpu_sampling(SAMPLING_CPU_SECONDS),
-                      "tensor": _cpu_tensor()}))
+                      "tensor": _cpu_tensor(), "gradient": _cpu_gradient(GRADIENT_CPU_SECONDS),
+                      "laplace_us": _cpu_laplace()}))
+
+
+def _cpu_gradient(seconds: float):
+    """The reference's grad_perm (oracle/_ref) on the gradient workload's
+    items in order until `seconds` are used."""
+    from oracle import oracle as O
+    from piquasso_amd import workloads as W
+
+    mats, rows, cols = W.gradient_batch(count=256, seed=5)
+    try:
+        O.ref_lib()
+        fn, kind = O.ref_grad, "reference"
+    except O.OracleUnavailable:
+        fn, kind = O.grad_dd, "port"
+    done = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds and done < len(mats):
+        fn(mats[done], rows[done], cols[done])
+        done += 1
+    return {"kind": kind, "items": done, "seconds": time.perf_counter() - t0}
+
+
+def _laplace_config():
+    """Column minors of the boson-sampling configuration's last photon: the
+    output pattern with one photon removed (19) against the 20 inputs."""
+    a, r, c, _ = _load_extended("boson_sampling_m40_n20")
+    r = np.array(r, dtype=np.int32)
+    r[int(np.flatnonzero(r)[0])] -= 1
+    return a, r, np.array(c, dtype=np.int32)
+
+
+def _cpu_laplace():
+    from oracle import oracle as O
+
+    a, r, c = _laplace_config()
+    try:
+        O.ref_lib()
+        return _median_call_us(lambda: O.ref_laplace(a, r, c), 5)
+    except O.OracleUnavailable:
+        return None
 
 
 def _cpu_tensor():
@@ -201,6 +243,8 @@ def run_cpu_baseline(seconds: float):
         "singles_us": rec.get("singles_us"),
         "sampling": rec.get("sampling"),
         "tensor": rec.get("tensor"),
+        "gradient": rec.get("gradient"),
+        "laplace_us": rec.get("laplace_us"),
     }
 
 
@@ -220,7 +264,7 @@ def _traffic_from_profile():
     return None
 
 
-def _gradient_leg(pa, W, reps: int):
+def _gradient_leg(pa, W, reps: int, cpu=None):
     """Gradients of 256 permanents, n ~ U{12..20}, unit multiplicities, Haar
     48-mode submatrices: forward sweep + one column-minor sweep per row, all
     in one hipGraph.  Euler's identity sum_ij A_ij G_ij = n perm(A) checks
@@ -238,7 +282,15 @@ def _gradient_leg(pa, W, reps: int):
         n = int(np.sum(r))
         resid = max(resid, abs(np.sum(a * g) - n * p) / np.sum(np.abs(a * g)))
     med = float(np.median(ms))
+    cg = (cpu or {}).get("gradient")
+    ref_cpu = None
+    if cg and cg.get("items"):
+        ref_cpu = {"gradients_per_s": cg["items"] / cg["seconds"], "kind": cg["kind"],
+                   "cores": (cpu or {}).get("cores"),
+                   "sample": f"the first {cg['items']} items of the 256, in order, "
+                             f"{cg['seconds']:.1f} s, grad_perm (n^2 permanents each)"}
     return {
+        "reference_cpu": ref_cpu,
         "gradients_per_s": len(mats) / (med * 1e-3),
         "ms": med,
         "tflops": st["flops"] / (med * 1e-3) / 1e12,
@@ -290,7 +342,8 @@ def _sampling_leg(pa, W, cpu):
     from piquasso_amd import sampling as PS
 
     u, occ = W.sampling()
-    pa.generate_samples(occ, 16, u, seed_sequence=SAMPLING_SEED)  # warm-up
+    # warm-up at full size (device and pinned buffers grow to the round size)
+    pa.generate_samples(occ, SAMPLING_SHOTS, u, seed_sequence=SAMPLING_SEED + SAMPLING_SHOTS)
     t0 = time.perf_counter()
     samples = pa.generate_samples(occ, SAMPLING_SHOTS, u, seed_sequence=SAMPLING_SEED)
     wall = time.perf_counter() - t0
@@ -474,7 +527,7 @@ def main():
                 "packed FP32, which this build does not emit",
     }
 
-    grad = _gradient_leg(pa, W, reps=max(3, min(args.steps, 10)))
+    grad = _gradient_leg(pa, W, reps=max(3, min(args.steps, 10)), cpu=cpu)
     sampling = _sampling_leg(pa, W, cpu)
     tensor = _tensor_leg(pa, W, cpu)
 
@@ -500,6 +553,19 @@ def main():
         if cpu and cpu.get("singles_us"):
             rec_s["reference_cpu_us"] = cpu["singles_us"].get(cid)
         singles[cid] = rec_s
+    # column minors (permanent_laplace) of the boson-sampling configuration
+    la, lr, lc = _laplace_config()
+    from oracle import oracle as O
+
+    lap = pa.permanent_laplace(la, lr, lc)
+    lap_ref = O.laplace_dd(la, lr, lc)
+    singles["laplace_m40_n20"] = {
+        "what": "permanent_laplace: the boson-sampling configuration with one output photon "
+                "removed (19 rows, 20 input columns): all 20 column minors",
+        "gpu_call_us": _median_call_us(lambda: pa.permanent_laplace(la, lr, lc), 20),
+        "rel_err": float(np.abs(lap - lap_ref).max() / np.abs(lap_ref).max()),
+        "reference_cpu_us": (cpu or {}).get("laplace_us"),
+    }
 
     line = {
         "metric": METRIC,
@@ -552,7 +618,8 @@ def main():
             "flop_model": "sum over items of T * (2 m_c + 6 (n - 1) + 4), T = Gray terms",
             "units": stats["units"],
         },
-        "cpu_baseline": {k: v for k, v in cpu.items() if k not in ("sampling", "tensor")} if cpu else cpu,
+        "cpu_baseline": ({k: v for k, v in cpu.items()
+                          if k not in ("sampling", "tensor", "gradient", "laplace_us")} if cpu else cpu),
     }
     print(json.dumps(line))
     return 0

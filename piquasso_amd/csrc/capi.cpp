@@ -318,7 +318,7 @@ int enqueue(Exec &x, int mode, hipEvent_t ev_begin, hipEvent_t ev_end) {
     if (p.all_direct && !p.minors) {
       // every item was a single unit that wrote its own result
     } else if (p.minors)
-      PQP_HIP(launch_minors_finalize(a, n_items, x.stream));
+      PQP_HIP(launch_minors_finalize(a, n_items, p.max_units, x.stream));
     else
       PQP_HIP(launch_finalize(comp, a, 0, n_items, static_cast<int>(p.ranges.size()),
                               p.max_units, x.stream));

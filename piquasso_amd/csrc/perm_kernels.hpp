@@ -56,7 +56,8 @@ hipError_t launch_minors_b7(const LaunchArgs &a, int first_unit, int n_units, hi
 hipError_t launch_minors_b8(const LaunchArgs &a, int first_unit, int n_units, hipStream_t s);
 hipError_t launch_minors(int bucket, const LaunchArgs &a, int first_unit, int n_units,
                          hipStream_t stream);
-hipError_t launch_minors_finalize(const LaunchArgs &a, int n_items, hipStream_t stream);
+hipError_t launch_minors_finalize(const LaunchArgs &a, int n_items, int max_units,
+                                  hipStream_t stream);
 
 // Tensor permanents (tensor_perm.hip): one launch per matrix order n over
 // units [first_unit, first_unit + n_units), then one finalize for all items.

@@ -280,40 +280,54 @@ __attribute__((amdgpu_waves_per_eu(minor_waves(HI, ONES), 8))) void perm_minors_
 }
 
 #if PQP_BUCKET_ID == 0
-// One wave per item; lane l sums column l's unit partials in unit order.
-__global__ __launch_bounds__(kWave) void perm_minors_finalize_kernel(LaunchArgs a) {
+// One block of NT threads per item.  For each column slot, thread t sums
+// the partials of units t, t + NT, ... in double-double, then the block
+// combines the thread sums in a fixed order (butterfly within each wave,
+// waves in index order), so the result does not depend on timing.
+template <int NT>
+__global__ __launch_bounds__(NT) void perm_minors_finalize_kernel(LaunchArgs a) {
+  constexpr int kWaves = NT / kWave;
+  __shared__ cdd wave_sum[kWaves];
   const int item = static_cast<int>(blockIdx.x);
-  const int lane = static_cast<int>(threadIdx.x);
+  const int tid = static_cast<int>(threadIdx.x);
   if (item >= a.n_items) return;
   const cptr<ItemDesc> it = as_const<ItemDesc>(a.items) + item;
   const int mc = it->mc;
   const int mcp = it->mcp;
   const int units = it->units;
   const int64_t base = it->partial_base;
-  for (int l = lane; l <= mc; l += kWave) {
+  const int e = it->n > 0 ? -(it->n - 1) : 0;
+  for (int l = 0; l <= mc; ++l) {
     // kernel slot mcp holds the uncut product; slots mc..mcp-1 are padding
     const int slot = (l == mc) ? mcp : l;
     cdd s = cdd_zero();
     if (it->n == 0) {
-      s.re.hi = 1.0;  // perm of an empty selection
+      if (tid == 0) s.re.hi = 1.0;  // perm of an empty selection
     } else {
-      for (int k = 0; k < units; ++k) {
+      for (int k = tid; k < units; k += NT) {
         const double *p = a.partials + 4 * (base + static_cast<int64_t>(k) * (mcp + 1) + slot);
         s = cdd_add(s, cdd{{p[0], p[1]}, {p[2], p[3]}});
       }
     }
-    const int e = it->n > 0 ? -(it->n - 1) : 0;
-    const double re_hi = ldexp(s.re.hi, e), re_lo = ldexp(s.re.lo, e);
-    const double im_hi = ldexp(s.im.hi, e), im_lo = ldexp(s.im.lo, e);
-    const int64_t o = static_cast<int64_t>(it->out_index) + l;
-    a.out[2 * o] = re_hi + re_lo;
-    a.out[2 * o + 1] = im_hi + im_lo;
-    if (a.out_dd) {
-      a.out_dd[4 * o] = re_hi;
-      a.out_dd[4 * o + 1] = re_lo;
-      a.out_dd[4 * o + 2] = im_hi;
-      a.out_dd[4 * o + 3] = im_lo;
+    s = wave_reduce(s);
+    if ((tid & (kWave - 1)) == 0) wave_sum[tid / kWave] = s;
+    __syncthreads();
+    if (tid == 0) {
+      cdd t = wave_sum[0];
+      for (int w = 1; w < kWaves; ++w) t = cdd_add(t, wave_sum[w]);
+      const double re_hi = ldexp(t.re.hi, e), re_lo = ldexp(t.re.lo, e);
+      const double im_hi = ldexp(t.im.hi, e), im_lo = ldexp(t.im.lo, e);
+      const int64_t o = static_cast<int64_t>(it->out_index) + l;
+      a.out[2 * o] = re_hi + re_lo;
+      a.out[2 * o + 1] = im_hi + im_lo;
+      if (a.out_dd) {
+        a.out_dd[4 * o] = re_hi;
+        a.out_dd[4 * o + 1] = re_lo;
+        a.out_dd[4 * o + 2] = im_hi;
+        a.out_dd[4 * o + 3] = im_lo;
+      }
     }
+    __syncthreads();  // wave_sum is rewritten for the next slot
   }
 }
 
@@ -364,9 +378,14 @@ hipError_t PQP_PASTE(launch_minors_b, PQP_BUCKET_ID)(const LaunchArgs &a, int fi
 }
 
 #if PQP_BUCKET_ID == 0
-hipError_t launch_minors_finalize(const LaunchArgs &a, int n_items, hipStream_t stream) {
+hipError_t launch_minors_finalize(const LaunchArgs &a, int n_items, int max_units,
+                                  hipStream_t stream) {
   if (n_items <= 0) return hipSuccess;
-  hipLaunchKernelGGL(perm_minors_finalize_kernel, dim3(n_items), dim3(kWave), 0, stream, a);
+  // a few large items (single calls) get 256 threads each, many small ones a wave
+  if (max_units > kWave)
+    hipLaunchKernelGGL(perm_minors_finalize_kernel<256>, dim3(n_items), dim3(256), 0, stream, a);
+  else
+    hipLaunchKernelGGL(perm_minors_finalize_kernel<kWave>, dim3(n_items), dim3(kWave), 0, stream, a);
   return hipGetLastError();
 }
 #endif
