@@ -280,10 +280,13 @@ __attribute__((amdgpu_waves_per_eu(minor_waves(HI, ONES), 8))) void perm_minors_
 }
 
 #if PQP_BUCKET_ID == 0
-// One block of NT threads per item.  For each column slot, thread t sums
-// the partials of units t, t + NT, ... in double-double, then the block
-// combines the thread sums in a fixed order (butterfly within each wave,
-// waves in index order), so the result does not depend on timing.
+// One block of NT threads per item.  With NT = 64 (items of at most 64
+// units, as in batches) lane l sums column slot l's unit partials in unit
+// order.  With NT = 256 (a few items of many units, as in single calls),
+// for each slot thread t sums the partials of units t, t + NT, ... in
+// double-double and the block combines the thread sums in a fixed order
+// (butterfly within each wave, waves in index order).  Both orders are
+// fixed, so results do not depend on timing.
 template <int NT>
 __global__ __launch_bounds__(NT) void perm_minors_finalize_kernel(LaunchArgs a) {
   constexpr int kWaves = NT / kWave;
@@ -297,6 +300,33 @@ __global__ __launch_bounds__(NT) void perm_minors_finalize_kernel(LaunchArgs a) 
   const int units = it->units;
   const int64_t base = it->partial_base;
   const int e = it->n > 0 ? -(it->n - 1) : 0;
+  if constexpr (NT == kWave) {
+    // few units per item (batches): lane l sums slot l's partials in unit order
+    for (int l = tid; l <= mc; l += kWave) {
+      const int slot = (l == mc) ? mcp : l;
+      cdd t = cdd_zero();
+      if (it->n == 0) {
+        t.re.hi = 1.0;
+      } else {
+        for (int k = 0; k < units; ++k) {
+          const double *p = a.partials + 4 * (base + static_cast<int64_t>(k) * (mcp + 1) + slot);
+          t = cdd_add(t, cdd{{p[0], p[1]}, {p[2], p[3]}});
+        }
+      }
+      const double re_hi = ldexp(t.re.hi, e), re_lo = ldexp(t.re.lo, e);
+      const double im_hi = ldexp(t.im.hi, e), im_lo = ldexp(t.im.lo, e);
+      const int64_t o = static_cast<int64_t>(it->out_index) + l;
+      a.out[2 * o] = re_hi + re_lo;
+      a.out[2 * o + 1] = im_hi + im_lo;
+      if (a.out_dd) {
+        a.out_dd[4 * o] = re_hi;
+        a.out_dd[4 * o + 1] = re_lo;
+        a.out_dd[4 * o + 2] = im_hi;
+        a.out_dd[4 * o + 3] = im_lo;
+      }
+    }
+    return;
+  }
   for (int l = 0; l <= mc; ++l) {
     // kernel slot mcp holds the uncut product; slots mc..mcp-1 are padding
     const int slot = (l == mc) ? mcp : l;
