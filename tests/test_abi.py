@@ -87,6 +87,39 @@ def test_argument_errors_precede_the_device():
         pa.permanent_laplace(np.eye(2), [1, 1], [1, 1])
 
 
+def test_batch_entry_errors_precede_the_device():
+    """The batched column-minor and tensor-permanent entries validate their
+    arguments before touching a device."""
+    import piquasso_amd as pa
+
+    L = _capi.lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    assert L.pqp_laplace_batch_c128(-1, None, None, None, None, None, None, None) == _capi.PQP_ERR_INVALID
+    assert L.pqp_laplace_batch_c128(1, None, None, None, None, None, None, None) == _capi.PQP_ERR_INVALID
+    assert L.pqp_laplace_batch_c128(0, None, None, None, None, None, None, None) == _capi.PQP_OK
+    with pytest.raises(RuntimeError):  # sum(cols) != sum(rows) + 1
+        pa.permanent_laplace_batch([np.eye(2)], [[1, 1]], [[1, 1]])
+    assert L.pqp_tensor_permanent_batch_c128(-1, None, None, None, None, None, None) == _capi.PQP_ERR_INVALID
+    a = np.zeros((2, 2), dtype=np.complex128)
+    mats = (dp * 1)(a.view(np.float64).ctypes.data_as(dp))
+    out = np.zeros(1)
+
+    def tensor(n, ld, x):
+        nn = np.array([n], dtype=np.int64)
+        ll = np.array([ld], dtype=np.int64)
+        xx = np.array([x])
+        return L.pqp_tensor_permanent_batch_c128(1, mats, nn.ctypes.data_as(i64p), ll.ctypes.data_as(i64p),
+                                                 xx.ctypes.data_as(dp), out.ctypes.data_as(dp), None)
+
+    assert tensor(21, 21, 0.5) == _capi.PQP_ERR_RANGE
+    assert tensor(-1, 2, 0.5) == _capi.PQP_ERR_INVALID
+    assert tensor(2, 1, 0.5) == _capi.PQP_ERR_INVALID
+    assert tensor(2, 2, float("nan")) == _capi.PQP_ERR_INVALID
+    with pytest.raises(ValueError):
+        pa.tensor_permanent(np.zeros((2, 3)), 0.5)
+
+
 def test_empty_selection_needs_no_device():
     """perm of an empty selection is 1 (src/permanent.cpp:106-108); the
     Laplace entry returns the reference's [1] for degenerate inputs."""
