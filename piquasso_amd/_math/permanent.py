@@ -43,6 +43,7 @@ number that is no permanent; here that raises ``RuntimeError``.
 from __future__ import annotations
 
 import ctypes
+import threading
 
 import numpy as np
 
@@ -74,10 +75,52 @@ def _multiplicities(values, name, length):
     return arr
 
 
+_c128_call = None
+_tls = threading.local()
+
+
+def _fast_c128():
+    """pqp_permanent_c128 bound with plain address arguments (cheaper to
+    call than the pointer-typed binding)."""
+    global _c128_call
+    if _c128_call is None:
+        c = ctypes
+        proto = c.CFUNCTYPE(c.c_int, c.c_void_p, c.c_int64, c.c_int64, c.c_int64, c.c_void_p,
+                            c.c_void_p, c.c_uint32, c.c_void_p, c.c_void_p)
+        _c128_call = proto(("pqp_permanent_c128", _capi.lib()))
+    return _c128_call
+
+
+def _out_buffer():
+    buf = getattr(_tls, "out", None)
+    if buf is None:
+        buf = np.zeros(2)
+        _tls.out = buf
+        _tls.out_addr = buf.ctypes.data
+    return buf, _tls.out_addr
+
+
 def permanent(matrix, rows, cols):
     """Permanent of ``matrix`` with row multiplicities ``rows`` and column
     multiplicities ``cols`` (Glynn's formula over the reflected n-ary Gray
     code, src/permanent.cpp:49-264), evaluated on the GPU."""
+    if (type(matrix) is np.ndarray and matrix.dtype == np.complex128 and matrix.ndim == 2
+            and matrix.flags.c_contiguous and type(rows) is np.ndarray and type(cols) is np.ndarray
+            and rows.ndim == 1 and cols.ndim == 1 and rows.dtype.kind in "iu" and cols.dtype.kind in "iu"):
+        # the common call: a C-contiguous complex128 matrix and integer
+        # arrays; same result, less marshalling
+        m_r, m_c = matrix.shape
+        if rows.shape[0] != m_r or cols.shape[0] != m_c:
+            raise ValueError(f"permanent(): rows / cols have {rows.shape[0]} / {cols.shape[0]} "
+                             f"entries for a {m_r} x {m_c} matrix")
+        r = rows if (rows.dtype == np.int32 and rows.flags.c_contiguous) else rows.astype(np.int32)
+        c = cols if (cols.dtype == np.int32 and cols.flags.c_contiguous) else cols.astype(np.int32)
+        out, out_addr = _out_buffer()
+        st = _fast_c128()(matrix.ctypes.data, m_r, m_c, m_c, r.ctypes.data, c.ctypes.data, 0,
+                          out_addr, None)
+        if st:
+            _capi.check(st)
+        return np.array(complex(out[0], out[1]), dtype=np.complex128)
     dtype = resolve_dtype(matrix)
     a = np.ascontiguousarray(np.asarray(matrix, dtype=dtype))
     if a.ndim != 2:

@@ -160,12 +160,45 @@ struct HostBuf {
   }
 };
 
+// Pinned host memory mapped into the device's address space: kernels write
+// small results straight into it, so a call needs no device-to-host copy
+// (the stream synchronisation makes the writes visible).
+struct MappedBuf {
+  void *host = nullptr;
+  void *dev = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t n) {
+    if (n <= bytes) return PQP_OK;
+    release();
+    size_t alloc = std::max<size_t>(n, 4096);
+    hipError_t e = hipHostMalloc(&host, alloc, hipHostMallocMapped | hipHostMallocCoherent);
+    if (e != hipSuccess) {
+      host = nullptr;
+      return fail(PQP_ERR_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    }
+    e = hipHostGetDevicePointer(&dev, host, 0);
+    if (e != hipSuccess) {
+      release();
+      return fail(PQP_ERR_DEVICE, std::string("hipHostGetDevicePointer: ") + hipGetErrorString(e));
+    }
+    bytes = alloc;
+    return PQP_OK;
+  }
+  void release() {
+    if (host) (void)hipHostFree(host);
+    host = dev = nullptr;
+    bytes = 0;
+  }
+};
+
 // Device image of one plan plus the stream it runs on.
 struct Exec {
   Plan plan;
   std::vector<int32_t> item_first_unit;  // first unit of each item in plan.units
   DevBuf arena, partials, heads;
   DevBuf results;   // 2 doubles per result, then 4 (dd parts) per result
+  bool mapped_results = false;  // results in results_m (host memory) instead
+  MappedBuf results_m;
   HostBuf stage;    // pinned staging for the arena upload and result downloads
   size_t n_results = 0;
   bool heads_clean = false;  // queue heads zeroed since (re)allocation
@@ -183,8 +216,9 @@ struct Exec {
     a.items = reinterpret_cast<const ItemDesc *>(static_cast<const char *>(arena.p) + plan.items_off);
     a.units = reinterpret_cast<const WorkUnit *>(static_cast<const char *>(arena.p) + plan.units_off);
     a.partials = static_cast<double *>(partials.p);
-    a.out = static_cast<double *>(results.p);
-    a.out_dd = static_cast<double *>(results.p) + 2 * n_results;
+    double *r = static_cast<double *>(mapped_results ? results_m.dev : results.p);
+    a.out = r;
+    a.out_dd = r + 2 * n_results;
     a.queue_head = static_cast<int *>(heads.p);
     a.n_units = static_cast<int32_t>(plan.units.size());
     a.n_items = static_cast<int32_t>(plan.items.size());
@@ -207,7 +241,11 @@ int upload(Exec &x) {
   int st;
   if ((st = x.arena.ensure(p.arena.size())) != PQP_OK) return st;
   if ((st = x.partials.ensure(std::max<int64_t>(p.n_partials, 1) * 4 * sizeof(double))) != PQP_OK) return st;
-  if ((st = x.results.ensure(n_out * 6 * sizeof(double))) != PQP_OK) return st;
+  if (x.mapped_results) {
+    if ((st = x.results_m.ensure(n_out * 6 * sizeof(double))) != PQP_OK) return st;
+  } else if ((st = x.results.ensure(n_out * 6 * sizeof(double))) != PQP_OK) {
+    return st;
+  }
   x.n_results = n_out;
   const void *heads_before = x.heads.p;
   if ((st = x.heads.ensure(64 * sizeof(int))) != PQP_OK) return st;
@@ -350,8 +388,10 @@ namespace {
 // reused across calls; calls serialise on a mutex.
 struct SingleState {
   std::mutex mu;
-  Exec x;
+  Exec x;       // column minors, gradients, tensor permanents
   bool init = false;
+  Exec xs;      // single permanents: results in mapped host memory
+  bool init_s = false;
 };
 
 SingleState &single_state() {
@@ -377,11 +417,12 @@ int run_single(const PermInput &in, uint32_t flags, double *out2, double *out_dd
   if ((st = require_device()) != PQP_OK) return st;
   SingleState &S = single_state();
   std::lock_guard<std::mutex> lock(S.mu);
-  Exec &x = S.x;
-  if (!S.init) {
+  Exec &x = S.xs;
+  if (!S.init_s) {
     PQP_HIP(hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking));
     x.own_stream = true;
-    S.init = true;
+    x.mapped_results = true;
+    S.init_s = true;
   }
   x.plan = std::move(plan);
   index_units(x);
@@ -390,12 +431,11 @@ int run_single(const PermInput &in, uint32_t flags, double *out2, double *out_dd
   if ((st = enqueue(x, (flags & PQP_FLAG_PLAIN) ? PQP_RUN_PLAIN : PQP_RUN_EAGER, nullptr,
                     nullptr)) != PQP_OK)
     return st;
-  // one item: its 2 result doubles and 4 dd parts are contiguous
-  // (results = [out (2 per result)][out_dd (4 per result)], one result)
-  if ((st = x.stage.ensure(6 * sizeof(double))) != PQP_OK) return st;
-  double *res = static_cast<double *>(x.stage.p);
-  PQP_HIP(hipMemcpyAsync(res, x.results.p, 6 * sizeof(double), hipMemcpyDeviceToHost, x.stream));
   PQP_HIP(hipStreamSynchronize(x.stream));
+  // one item: its 2 result doubles and 4 dd parts are contiguous
+  // (results = [out (2 per result)][out_dd (4 per result)], one result),
+  // written by the kernels into mapped host memory
+  const double *res = static_cast<const double *>(x.results_m.host);
   out2[0] = res[0];
   out2[1] = res[1];
   if (out_dd4) std::memcpy(out_dd4, res + 2, 4 * sizeof(double));

@@ -279,3 +279,26 @@ def test_column_power_forms(gpu, oracle, cols):
     # the same item inside a batch (another launch path) gives the same value
     vals = pa.permanent_batch([a, a], [r, r], [c, c])
     assert abs(vals[0] - ref) <= tol and vals[0] == vals[1]
+
+
+def test_drop_in_argument_forms_agree(gpu):
+    """The lean complex128 path and the general path give the same value for
+    every argument form (strided / typed multiplicities, lists, views)."""
+    import piquasso_amd as pa
+
+    rng = np.random.default_rng(12)
+    a = (rng.standard_normal((7, 7)) + 1j * rng.standard_normal((7, 7))) / np.sqrt(14)
+    r = np.array([1, 0, 2, 1, 0, 1, 1])
+    c = np.array([2, 1, 0, 1, 1, 0, 1])
+    ref = complex(pa.permanent(a, r.tolist(), c.tolist()))
+    forms = [
+        (a, r, c),
+        (a, r.astype(np.int32), c.astype(np.uint8)),
+        (a, np.repeat(r, 2)[::2], np.repeat(c, 2)[::2]),  # strided multiplicities
+        (np.asfortranarray(a), r, c),                        # general path
+        (a, list(r), tuple(c)),
+    ]
+    for m, rr, cc in forms:
+        v = pa.permanent(m, rr, cc)
+        assert v.shape == () and v.dtype == np.complex128
+        assert abs(complex(v) - ref) <= 1e-13 * abs(ref)
