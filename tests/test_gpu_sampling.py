@@ -156,3 +156,20 @@ def test_sampler_chunked_minors_calls(gpu, monkeypatch):
     monkeypatch.setattr(P._MinorsBatch, "max_problems", 7)
     got = pa.generate_samples(occ, 40, u, seed_sequence=5)
     assert got == want
+
+
+def test_sampler_edge_cases(gpu):
+    """No photons, one shot, every photon in one mode, a single mode, and the
+    identity interferometer (photons stay where they are)."""
+    import piquasso_amd as pa
+    from oracle import sampling as S
+
+    u = _haar(5, 3)
+    assert pa.generate_samples([0, 0, 0, 0, 0], 4, u, seed_sequence=1) == [(0, 0, 0, 0, 0)] * 4
+    for occ in ([4, 0, 0, 0, 0], [0, 1, 0, 0, 0], [1, 1, 1, 1, 1]):
+        got = pa.generate_samples(occ, 1, u, seed_sequence=9)
+        assert got == S.generate_samples(occ, 1, u, 9)
+        assert sum(got[0]) == sum(occ)
+    assert pa.generate_samples([3], 3, np.ones((1, 1)), seed_sequence=2) == [(3,)] * 3
+    occ = [2, 0, 1, 3]
+    assert pa.generate_samples(occ, 5, np.eye(4), seed_sequence=4) == [tuple(occ)] * 5

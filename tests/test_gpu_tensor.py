@@ -102,3 +102,19 @@ def test_tensor_errors(gpu):
     with pytest.raises(ValueError):
         pa.detection_probabilities([[1, 0]], np.eye(2), [1, -1], 0.5)
     assert pa.tensor_permanent_batch([], []).shape == (0,)
+
+
+def test_tensor_permanent_n18_limits(gpu):
+    """n = 18 (4^17 / 2 pairs, ~0.06 s): the x = 1 and x = 0 limits against
+    the Glynn kernels."""
+    import piquasso_amd as pa
+    from piquasso_amd import workloads as W
+
+    a = W.tensor(18)
+    print("tensor permanent limits n=18", flush=True)
+    (t1, t0), (s1, s0) = pa.tensor_permanent_batch([a, a], [1.0, 0.0], with_abs_sum=True)
+    ones = np.ones(18, int)
+    p = complex(pa.permanent(a, ones, ones))
+    p2 = complex(pa.permanent(np.abs(a) ** 2, ones, ones))
+    assert abs(t1 - abs(p) ** 2) <= 8 * 20 * EPS * s1 + 1e-13 * abs(p) ** 2
+    assert abs(t0 - p2.real) <= 8 * 20 * EPS * s0 + 1e-13 * abs(p2)
