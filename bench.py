@@ -19,6 +19,7 @@ Printed as ONE JSON line (rank 0):
   tensor          tensor permanent of 18 partially distinguishable photons
                   (detection probabilities): time and % of FP64 peak; the CPU
                   figure is a numpy restatement of the reference's loop at n = 9
+  output_probabilities  a probability table (77 520 output patterns) in one call
   sampling        Clifford & Clifford boson samples/s (40 modes, 20 photons,
                   1024 shots in lockstep: one batched column-minor launch per
                   photon round), checked shot for shot against the reference's
@@ -56,6 +57,7 @@ SAMPLING_SEED = 1234
 SAMPLING_CPU_SECONDS = 8.0
 TENSOR_N = 18
 GRADIENT_CPU_SECONDS = 5.0
+PROB_CPU_SECONDS = 4.0
 TENSOR_CPU_N = 9
 
 
@@ -137,7 +139,28 @@ def cpu_baseline_child(seconds: float) -> None:
                       "threads": int(os.environ.get("OMP_THREAD_LIMIT", "0") or 0),
                       "singles_us": singles, "sampling": _cpu_sampling(SAMPLING_CPU_SECONDS),
                       "tensor": _cpu_tensor(), "gradient": _cpu_gradient(GRADIENT_CPU_SECONDS),
-                      "laplace_us": _cpu_laplace()}))
+                      "laplace_us": _cpu_laplace(), "probabilities": _cpu_probabilities(PROB_CPU_SECONDS)}))
+
+
+def _cpu_probabilities(seconds: float):
+    """The reference's way to a probability table: its pybind permanent()
+    once per output pattern (piquasso/_simulators/passive/probabilities.py:26-54),
+    patterns in order until `seconds` are used."""
+    from oracle import oracle as O
+    from piquasso_amd import workloads as W
+
+    u, inp, outs = W.output_patterns()
+    try:
+        fn, kind = O.ref_module().permanent, "reference"
+    except O.OracleUnavailable:
+        fn, kind = O.perm_port, "port"
+    u = np.ascontiguousarray(u)
+    done = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds and done < len(outs):
+        fn(u, outs[done], inp)
+        done += 1
+    return {"kind": kind, "patterns": done, "seconds": time.perf_counter() - t0}
 
 
 def _cpu_gradient(seconds: float):
@@ -247,6 +270,7 @@ def run_cpu_baseline(seconds: float):
         "tensor": rec.get("tensor"),
         "gradient": rec.get("gradient"),
         "laplace_us": rec.get("laplace_us"),
+        "probabilities": rec.get("probabilities"),
     }
 
 
@@ -333,6 +357,35 @@ def _tensor_leg(pa, W, cpu):
                       f"({ct['pairs']:.0f} pairs) in {ct['seconds']:.2f} s, one thread",
         }
         rec["gpu_pairs_per_s"] = 4.0 ** (TENSOR_N - 1) / 2 / t
+    return rec
+
+
+def _probabilities_leg(pa, W, cpu):
+    """A probability table: every output pattern of 7 photons in 14 modes
+    (77 520 permanents, n = 7) in one batched call, timed around the whole
+    call (planning, upload, kernels, download); the probabilities of a
+    unitary must sum to 1."""
+    u, inp, outs = W.output_patterns()
+    pa.output_probabilities(u, inp, outs[:1000])  # warm-up
+    t0 = time.perf_counter()
+    p = pa.output_probabilities(u, inp, outs)
+    t = time.perf_counter() - t0
+    rec = {
+        "patterns": len(outs),
+        "seconds": t,
+        "probabilities_per_s": len(outs) / t,
+        "sum_minus_one": float(p.sum() - 1.0),
+        "what": "output_probabilities: all 77520 output patterns of 7 photons in 14 modes of a Haar "
+                "unitary, one batched call",
+    }
+    cp = (cpu or {}).get("probabilities")
+    if cp and cp.get("patterns"):
+        rec["reference_cpu"] = {
+            "probabilities_per_s": cp["patterns"] / cp["seconds"],
+            "kind": cp["kind"],
+            "sample": f"the reference's permanent() per pattern, first {cp['patterns']} patterns in "
+                      f"{cp['seconds']:.1f} s",
+        }
     return rec
 
 
@@ -540,6 +593,7 @@ def main():
     grad = _gradient_leg(pa, W, reps=max(3, min(args.steps, 10)), cpu=cpu)
     sampling = _sampling_leg(pa, W, cpu)
     tensor = _tensor_leg(pa, W, cpu)
+    probabilities = _probabilities_leg(pa, W, cpu)
 
     # single permanents through the drop-in entry (host buffers in, result
     # out: planning, upload, sweep, download); the reference's CPU time for
@@ -616,6 +670,7 @@ def main():
         "gradient": grad,
         "sampling": sampling,
         "tensor_permanent": tensor,
+        "output_probabilities": probabilities,
         "single_permanents": singles,
         "roofline": {
             "bound": "mfma",
@@ -632,7 +687,8 @@ def main():
             "units": stats["units"],
         },
         "cpu_baseline": ({k: v for k, v in cpu.items()
-                          if k not in ("sampling", "tensor", "gradient", "laplace_us")} if cpu else cpu),
+                          if k not in ("sampling", "tensor", "gradient", "laplace_us", "probabilities")}
+                         if cpu else cpu),
     }
     print(json.dumps(line))
     return 0

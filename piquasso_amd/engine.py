@@ -63,6 +63,37 @@ def permanent_ex(matrix, rows, cols, mode: str = "dd"):
     return complex(out[0], out[1]), dd
 
 
+def _shared_marshal(matrices, rows, cols):
+    """Arguments of pqp_batch_create for a batch whose items all use the same
+    matrix object and whose multiplicities stack into (count, m_r) and
+    (count, m_c) integer arrays; None when the batch is not of that form."""
+    n = len(matrices)
+    if n < 2 or any(m is not matrices[0] for m in matrices):
+        return None
+    try:
+        a = _prep(matrices[0], np.zeros(np.shape(matrices[0])[0], np.int32),
+                  np.zeros(np.shape(matrices[0])[1], np.int32))[0]
+        R = np.asarray(rows)
+        C = np.asarray(cols)
+    except (ValueError, TypeError, IndexError):
+        return None
+    m_r, m_c = a.shape
+    if R.shape != (n, m_r) or C.shape != (n, m_c) or R.dtype.kind not in "iu" or C.dtype.kind not in "iu":
+        return None
+    R32 = np.ascontiguousarray(R, dtype=np.int32)
+    C32 = np.ascontiguousarray(C, dtype=np.int32)
+    idx = np.arange(n, dtype=np.uint64)
+    mats = np.full(n, a.ctypes.data, dtype=np.uint64)
+    rp = np.uint64(R32.ctypes.data) + idx * np.uint64(4 * m_r)
+    cp = np.uint64(C32.ctypes.data) + idx * np.uint64(4 * m_c)
+    keep = [a, R32, C32, mats, rp, cp]
+    mat_arr = mats.ctypes.data_as(ctypes.POINTER(ctypes.c_void_p))
+    row_arr = rp.ctypes.data_as(ctypes.POINTER(_i32p))
+    col_arr = cp.ctypes.data_as(ctypes.POINTER(_i32p))
+    return (keep, mat_arr, row_arr, col_arr, np.full(n, m_r, dtype=np.int64),
+            np.full(n, m_c, dtype=np.int64))
+
+
 class PermanentBatch:
     """A planned batch of permanents resident on the GPU.
 
@@ -127,20 +158,26 @@ class PermanentBatch:
             if not total_units:
                 total_units = int(tuned.get("total_units", 0))
         self.gradient = bool(gradient)
-        self._keep = []
-        mats, rws, cls = [], [], []
-        for m, r, c in zip(matrices, rows, cols):
-            a, rr, cc = _prep(m, r, c)
-            self._keep.extend((a, rr, cc))
-            mats.append(a.ctypes.data)
-            rws.append(rr.ctypes.data_as(_i32p))
-            cls.append(cc.ctypes.data_as(_i32p))
         n = self.count
-        mat_arr = (ctypes.c_void_p * max(n, 1))(*mats)
-        row_arr = (_i32p * max(n, 1))(*rws)
-        col_arr = (_i32p * max(n, 1))(*cls)
-        m_r = np.array([self._keep[3 * k].shape[0] for k in range(n)] or [0], dtype=np.int64)
-        m_c = np.array([self._keep[3 * k].shape[1] for k in range(n)] or [0], dtype=np.int64)
+        shared = _shared_marshal(matrices, rows, cols)
+        if shared is not None:
+            # one matrix for every item and multiplicities that stack into
+            # two int32 arrays (probability tables): addresses by arithmetic
+            self._keep, mat_arr, row_arr, col_arr, m_r, m_c = shared
+        else:
+            self._keep = []
+            mats, rws, cls = [], [], []
+            for m, r, c in zip(matrices, rows, cols):
+                a, rr, cc = _prep(m, r, c)
+                self._keep.extend((a, rr, cc))
+                mats.append(a.ctypes.data)
+                rws.append(rr.ctypes.data_as(_i32p))
+                cls.append(cc.ctypes.data_as(_i32p))
+            mat_arr = (ctypes.c_void_p * max(n, 1))(*mats)
+            row_arr = (_i32p * max(n, 1))(*rws)
+            col_arr = (_i32p * max(n, 1))(*cls)
+            m_r = np.array([self._keep[3 * k].shape[0] for k in range(n)] or [0], dtype=np.int64)
+            m_c = np.array([self._keep[3 * k].shape[1] for k in range(n)] or [0], dtype=np.int64)
         flags = 0
         if precision == "fp32":
             flags |= _capi.PQP_FLAG_FP32

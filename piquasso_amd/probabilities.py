@@ -46,13 +46,14 @@ def output_amplitudes(unitary, input_state: Sequence[int], output_states,
     ok = outs.sum(axis=1) == n  # other photon numbers have amplitude 0
     idx = np.flatnonzero(ok)
     if idx.size:
-        with PermanentBatch([u] * idx.size, [outs[k] for k in idx], [s] * idx.size,
-                            precision=precision) as b:
+        sel = np.ascontiguousarray(outs[idx], dtype=np.int32)
+        with PermanentBatch([u] * idx.size, list(sel), [s] * idx.size, precision=precision) as b:
             b.run("eager")
             perms = b.results()
-        s_norm = _norm(s)
-        for j, k in enumerate(idx):
-            amps[k] = perms[j] / math.sqrt(_norm(outs[k]) * s_norm)
+        # prod t! over the output pattern, from a factorial table (n <= 64)
+        fact = np.array([math.factorial(k) for k in range(n + 1)], dtype=np.float64)
+        out_norm = np.prod(fact[sel], axis=1)
+        amps[idx] = perms / np.sqrt(out_norm * _norm(s))
     return amps
 
 

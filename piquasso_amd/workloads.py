@@ -14,6 +14,9 @@ Workloads (sizes from the engine's benchmark definition):
   batched_uneven    4096 submatrices of a 48-mode Haar unitary, n ~ U{8..24},
                     row / column multiplicities in {1, 2, 3} summing to n
   gradient_batch    256 Haar submatrices, n ~ U{12..20}, multiplicities 1
+  output_patterns   every output occupation of `photons` photons in `modes`
+                    modes of a Haar unitary, input one photon in each of the
+                    first modes (probability tables)
   tensor            n x n i.i.d. (N + iN) / sqrt(2n) matrix for the tensor
                     permanent of partially distinguishable photons
   sampling          Clifford & Clifford sampling: 40-mode Haar unitary, one
@@ -36,6 +39,7 @@ __all__ = [
     "minors_flops",
     "tensor",
     "tensor_flops",
+    "output_patterns",
     "glynn_terms",
     "glynn_flops",
 ]
@@ -204,3 +208,19 @@ def tensor_flops(n: int) -> float:
     if n <= 0:
         return 0.0
     return 4.0 ** (n - 1) / 2.0 * (16.0 * n - 4.0)
+
+
+def output_patterns(modes: int = 14, photons: int = 7, seed: int = 8):
+    """(U, input occupation, all output occupations) for a probability table:
+    a Haar unitary, one photon in each of the first `photons` modes, and every
+    way of placing `photons` photons in `modes` modes (lexicographic)."""
+    import itertools
+
+    rng = np.random.default_rng(seed)
+    u = haar_unitary(modes, rng)
+    inp = np.zeros(modes, dtype=np.int64)
+    inp[:photons] = 1
+    outs = np.array([np.bincount(c, minlength=modes)
+                     for c in itertools.combinations_with_replacement(range(modes), photons)],
+                    dtype=np.int64)
+    return u, inp, outs

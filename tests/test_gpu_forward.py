@@ -302,3 +302,24 @@ def test_drop_in_argument_forms_agree(gpu):
         v = pa.permanent(m, rr, cc)
         assert v.shape == () and v.dtype == np.complex128
         assert abs(complex(v) - ref) <= 1e-13 * abs(ref)
+
+
+def test_batch_shared_matrix_marshalling(gpu):
+    """A batch that reuses one matrix object with stacked multiplicities
+    (the probability-table form) gives the same permanents as the same items
+    passed one by one."""
+    rng = np.random.default_rng(21)
+    u = W.haar_unitary(9, rng)
+    outs = np.array([np.bincount(c, minlength=9)
+                     for c in itertools.combinations_with_replacement(range(9), 4)])[::7]
+    inp = np.array([1, 1, 0, 1, 0, 0, 1, 0, 0])
+    with pa.PermanentBatch([u] * len(outs), outs, [inp] * len(outs)) as b:
+        b.run("eager")
+        shared = b.results()
+    copies = [u.copy() for _ in outs]  # distinct objects: the item-by-item path
+    with pa.PermanentBatch(copies, list(outs), [inp] * len(outs)) as b:
+        b.run("eager")
+        single = b.results()
+    np.testing.assert_array_equal(shared, single)
+    p = pa.output_probabilities(u, inp, outs)
+    assert p.shape == (len(outs),) and np.all(p >= 0)
