@@ -10,6 +10,7 @@ PermanentBatch / permanent_batch     many uneven permanents in one persistent la
 permanent_laplace(matrix, rows, cols) drop-in for piquasso._math.permanent.permanent_laplace
 permanent_grad / permanent_grad_batch d perm / d A (grad_perm, any shape)
 torch_permanent(A, rows, cols)       differentiable permanent (torch.autograd.Function)
+torch_permanent_batch(A, rows, cols) the same for a (B, m_r, m_c) stack, one engine call
 output_probabilities(U, s, outputs)  boson-sampling probabilities of many outputs in one batch
 permanent_laplace_batch(ms, rs, cs)  many permanent_laplace problems in one launch
 generate_samples / generate_lossy_samples  Clifford & Clifford sampling, shots in lockstep
@@ -18,7 +19,8 @@ tensor_permanent / detection_probabilities  uniformly partially distinguishable 
 
 from ._math.permanent import permanent, permanent_laplace, permanent_laplace_batch
 from .engine import PermanentBatch, permanent_batch, permanent_ex
-from .grad import PermanentFunction, permanent_grad, permanent_grad_batch, torch_permanent
+from .grad import (PermanentBatchFunction, PermanentFunction, permanent_grad, permanent_grad_batch,
+                   torch_permanent, torch_permanent_batch)
 from .probabilities import output_amplitudes, output_probabilities
 from .distinguishability import (detection_probabilities, detection_probability, tensor_permanent,
                                  tensor_permanent_batch)
@@ -29,6 +31,7 @@ __all__ = [
     "permanent_grad", "permanent_grad_batch", "PermanentFunction", "torch_permanent",
     "output_amplitudes", "output_probabilities", "permanent_laplace_batch",
     "generate_samples", "generate_lossy_samples", "tensor_permanent", "tensor_permanent_batch",
-    "detection_probability", "detection_probabilities",
+    "detection_probability", "detection_probabilities", "torch_permanent_batch",
+    "PermanentBatchFunction",
 ]
 __version__ = "0.1.0"

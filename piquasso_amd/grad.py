@@ -14,6 +14,10 @@ reference's routine assumes a square matrix; this one takes any m_r x m_c.
 
 * :func:`permanent_grad` — G for one permanent;
 * :func:`permanent_grad_batch` — permanents and G for many;
+* :class:`PermanentBatchFunction` / :func:`torch_permanent_batch` — the
+  same for a stack of matrices (B, m_r, m_c) with per-item multiplicities,
+  all permanents and gradients from one engine call (the batched backward
+  of the JAX extension, src/jax_perm/cuda/kernels.cc.cu:431-630);
 * :class:`PermanentFunction` / :func:`torch_permanent` — a
   ``torch.autograd.Function`` with PyTorch's convention for complex inputs:
   ``A.grad = grad_output * conj(G)`` (for the loss |perm|^2 this is
@@ -30,7 +34,8 @@ import numpy as np
 
 from . import _capi
 
-__all__ = ["permanent_grad", "permanent_grad_batch", "PermanentFunction", "torch_permanent"]
+__all__ = ["permanent_grad", "permanent_grad_batch", "PermanentFunction", "torch_permanent",
+           "PermanentBatchFunction", "torch_permanent_batch"]
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _i32p = ctypes.POINTER(ctypes.c_int32)
@@ -145,3 +150,64 @@ def torch_permanent(A, rows, cols):
     """Differentiable permanent of a complex torch tensor (see
     :class:`PermanentFunction`)."""
     return PermanentFunction.apply(A, rows, cols)
+
+
+class PermanentBatchFunction:
+    """``PermanentBatchFunction.apply(A, rows, cols)``: permanents of a stack
+    of complex matrices A (B, m_r, m_c) with multiplicities rows (B, m_r) and
+    cols (B, m_c) as one differentiable op; returns a (B,) tensor of A's
+    dtype on A's device.  With ``A.requires_grad`` the forward evaluates all
+    permanents and gradient matrices in one ``pqp_grad_batch_c128`` call and
+    the backward returns ``grad_output[:, None, None] * conj(G)``."""
+
+    _impl = None
+
+    @classmethod
+    def _cls(cls):
+        if cls._impl is None:
+            torch = _torch()
+
+            def _host(x):
+                return np.asarray(x.detach().cpu().numpy() if torch.is_tensor(x) else x)
+
+            class _PermBatch(torch.autograd.Function):
+                @staticmethod
+                def forward(ctx, A, rows, cols):
+                    if A.dim() != 3:
+                        raise ValueError(f"expected a (B, m_r, m_c) tensor, got shape {tuple(A.shape)}")
+                    a = A.detach().to("cpu", torch.complex128).numpy()
+                    r = _host(rows)
+                    c = _host(cols)
+                    B = a.shape[0]
+                    if r.shape != (B, a.shape[1]) or c.shape != (B, a.shape[2]):
+                        raise ValueError("rows / cols must have shapes (B, m_r) / (B, m_c)")
+                    if A.requires_grad:
+                        perms, grads = permanent_grad_batch(list(a), list(r), list(c))
+                        G = np.stack(grads) if B else np.zeros(a.shape, dtype=np.complex128)
+                        ctx.save_for_backward(torch.from_numpy(G).to(A.device, A.dtype))
+                    else:
+                        from .engine import permanent_batch
+
+                        perms = permanent_batch(list(a), list(r), list(c)) if B else np.zeros(0, complex)
+                        ctx.save_for_backward(None)
+                    return torch.from_numpy(np.asarray(perms, dtype=np.complex128)).to(A.device, A.dtype)
+
+                @staticmethod
+                def backward(ctx, grad_out):
+                    (G,) = ctx.saved_tensors
+                    if G is None:
+                        return None, None, None
+                    return grad_out[:, None, None] * G.conj(), None, None
+
+            cls._impl = _PermBatch
+        return cls._impl
+
+    @classmethod
+    def apply(cls, A, rows, cols):
+        return cls._cls().apply(A, rows, cols)
+
+
+def torch_permanent_batch(A, rows, cols):
+    """Differentiable permanents of a stack of complex matrices (see
+    :class:`PermanentBatchFunction`)."""
+    return PermanentBatchFunction.apply(A, rows, cols)
