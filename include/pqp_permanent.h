@@ -199,6 +199,27 @@ int pqp_tensor_permanent_batch_c128(int64_t count, const double *const *matrices
                                     double *abs_sum);
 
 /*
+ * Coefficients of the permanent of a matrix polynomial,
+ *   out[q] = [x^e_q] perm(B + sum_{m < n_vars} x_m C_m)   (2 doubles: re, im)
+ * for n x n complex matrices B (b0) and C_0..C_{n_vars-1} (bm: n_vars
+ * matrices one after another; row-major, interleaved complex) and the
+ * exponent rows e_q (exponents: n_out x n_vars int32, row-major).  This is
+ * the subset loop of get_lossy_partially_distinguishable_detection_probabilities
+ * (piquasso/_simulators/passive/probabilities.py:341-382): Ryser's formula
+ * over the 2^n - 1 non-empty column subsets, each subset's product of row
+ * factors kept as a polynomial; here on the monomials that divide some e_q
+ * (the reference keeps the box of the largest exponent per variable; the
+ * kept coefficients are the same).  Rows with total degree above n give 0.
+ * info (may be null, 6 entries): monomials, links, workgroups, 1 if the
+ * polynomial was kept in LDS, algorithmic FP64 flops per subset, subsets.
+ * n <= 24, at most 255 variables with a non-zero exponent, n x (those
+ * variables + 1) <= 3072 and at most 2^23 monomials (PQP_ERR_RANGE beyond).
+ */
+int pqp_perm_poly_coefficients_c128(int64_t n, int64_t n_vars, const double *b0, const double *bm,
+                                    int64_t n_out, const int32_t *exponents, double *out,
+                                    int64_t *info);
+
+/*
  * Gradients of `count` permanents: grads[k] (m_r[k] x m_c[k] complex,
  * row-major, interleaved) = d perm(A_k; r_k, c_k) / d (A_k)_ij
  * = r_i c_j perm(A_k; r_k - e_i, c_k - e_j) (0 where r_i or c_j is 0), the

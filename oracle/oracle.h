@@ -81,6 +81,18 @@ int pqo_num_procs(void);
 int pqo_tensor_perm_dd(const double *a, int n, double x, int n_jobs, double *out,
                        double *abs_sum);
 
+/* Coefficients of the permanent of a matrix polynomial,
+ * out[p] = [x^e_p] perm(B_0 + sum_m x_m B_m), by Ryser's formula with the
+ * polynomial of every column subset truncated to the box of the requested
+ * exponents (piquasso/_simulators/passive/probabilities.py:341-382), all in
+ * double-double.  b0: n x n, bm: n_vars matrices n x n one after another
+ * (interleaved complex); exps: n_out x n_vars; out: (re, im) per exponent;
+ * abs_out: sum over subsets of the absolute-value polynomial's coefficient
+ * (a scale for forward-error bounds).  n <= 20; -2 on invalid arguments,
+ * -3 when the box exceeds 2^26 coefficients or memory runs out. */
+int pqo_perm_poly_dd(const double *b0, const double *bm, int n, int n_vars, const int32_t *exps,
+                     int n_out, double *out, double *abs_out);
+
 #ifdef __cplusplus
 }
 #endif

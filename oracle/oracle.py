@@ -20,7 +20,12 @@ Contents
   - :func:`laplace_dd` — ``permanent_laplace_cpp`` (src/permanent_laplace.cpp:43-240)
     as separate double-double sums per column;
   - :func:`gray_states` — the reflected n-ary Gray counter
-    (src/n_aryGrayCodeCounter.hpp:170-254).
+    (src/n_aryGrayCodeCounter.hpp:170-254);
+  - :func:`perm_poly_dd` / :func:`lossy_pd_probabilities_dd` — the Ryser
+    subset loop of the lossy partially distinguishable detection
+    probabilities (piquasso/_simulators/passive/probabilities.py:244-426) in
+    double-double, pinned by :func:`perm_poly_bruteforce` (the permutation
+    sum with polynomial entries).
 
 * independent pure-Python / numpy formulas used to pin the above:
 
@@ -82,6 +87,7 @@ def lib():
             ("pqo_laplace_dd_c128", [dp, ci, ci, ip, ip, ci, dp]),
             ("pqo_gray_states", [ip, ci, ctypes.c_int64, ctypes.c_int64, ip]),
             ("pqo_tensor_perm_dd", [dp, ci, ctypes.c_double, ci, dp, dp]),
+            ("pqo_perm_poly_dd", [dp, dp, ci, ci, ctypes.POINTER(ctypes.c_int32), ci, dp, dp]),
         ):
             fn = getattr(L, name)
             fn.argtypes = argt
@@ -286,6 +292,128 @@ def pd_detection_probability(occupation, interferometer, input_occupation, overl
     t, _ = tensor_perm_dd(a, overlap)
     den = uniform_input_norm(inp, overlap) * float(np.prod([math.factorial(int(k)) for k in occ]))
     return t.real / den
+
+
+def perm_poly_dd(b0, bms, exponents):
+    """Coefficients [x^e] perm(B_0 + sum_m x_m B_m) for the rows e of
+    ``exponents`` (shape (count, len(bms))), by the reference's Ryser subset
+    loop with box-truncated polynomials
+    (piquasso/_simulators/passive/probabilities.py:341-382) in double-double.
+    Returns (coefficients as complex array, sum over subsets of the
+    absolute-value polynomial's coefficient)."""
+    b0 = np.ascontiguousarray(np.asarray(b0, dtype=np.complex128))
+    n = b0.shape[0]
+    if b0.shape != (n, n):
+        raise ValueError("B_0 must be square")
+    bm = np.ascontiguousarray(np.asarray(bms, dtype=np.complex128).reshape(-1, n, n))
+    m = bm.shape[0]
+    e = np.ascontiguousarray(np.atleast_2d(np.asarray(exponents, dtype=np.int32)))
+    if e.shape[1] != m:
+        raise ValueError("one exponent per variable")
+    out = np.zeros(2 * max(len(e), 1))
+    asum = np.zeros(max(len(e), 1))
+    _check(lib().pqo_perm_poly_dd(
+        _ptr(b0.view(np.float64), ctypes.c_double),
+        _ptr(bm.view(np.float64), ctypes.c_double) if m else None, n, m,
+        _ptr(e, ctypes.c_int32), len(e), _ptr(out, ctypes.c_double), _ptr(asum, ctypes.c_double)))
+    return out[: 2 * len(e)].view(np.complex128).copy(), asum[: len(e)].copy()
+
+
+def perm_poly_bruteforce(b0, bms, exponent) -> complex:
+    """[x^e] perm(B_0 + sum_m x_m B_m) from the definition: the sum over
+    permutations sigma of prod_i (B_0 + sum_m x_m B_m)[i, sigma(i)], each
+    product expanded as a polynomial (dict from exponent tuples) and the
+    coefficient of x^e read off.  Independent of Ryser's formula; n <= 6."""
+    b0 = np.asarray(b0, dtype=np.complex128)
+    bm = np.asarray(bms, dtype=np.complex128).reshape(-1, *b0.shape)
+    n, m = b0.shape[0], bm.shape[0]
+    target = tuple(int(v) for v in exponent)
+    total = 0j
+    for sigma in itertools.permutations(range(n)):
+        poly = {(0,) * m: 1.0 + 0j}
+        for i in range(n):
+            j = sigma[i]
+            nxt: dict = {}
+            for key, val in poly.items():
+                nxt[key] = nxt.get(key, 0j) + val * b0[i, j]
+                for v in range(m):
+                    k2 = key[:v] + (key[v] + 1,) + key[v + 1:]
+                    if k2[v] <= target[v]:
+                        nxt[k2] = nxt.get(k2, 0j) + val * bm[v, i, j]
+            poly = nxt
+        total += poly.get(target, 0j)
+    return total
+
+
+def lossy_pd_matrices(transmission_matrix, input_occupation, particle_overlap,
+                      conjugate_loss: bool = False):
+    """B_loss and the B_m of the lossy partially distinguishable detection
+    probabilities (probabilities.py:304-338): G = x J + (1 - x) I for a
+    scalar overlap x or the Gram matrix itself, B_loss = G o (I - T^dag T)
+    on the first-quantized input modes, B_m = G o (t_m t_m^dag) with
+    t_m = T[m, input_modes].  ``conjugate_loss`` takes the complex conjugate
+    of I - T^dag T, the kernel that matches sum_m t_m t_m^dag = (T^dag T)^T."""
+    t = np.asarray(transmission_matrix, dtype=np.complex128)
+    modes = np.repeat(np.arange(len(input_occupation)), np.asarray(input_occupation, dtype=int))
+    n = len(modes)
+    if np.isscalar(particle_overlap):
+        x = float(particle_overlap)
+        g = x * np.ones((n, n), dtype=np.complex128) + (1.0 - x) * np.eye(n)
+    else:
+        g = np.asarray(particle_overlap, dtype=np.complex128)
+    kernel = (np.eye(t.shape[1]) - t.conj().T @ t)[np.ix_(modes, modes)]
+    if conjugate_loss:
+        kernel = np.conj(kernel)
+    b0 = g * kernel
+    bms = np.array([g * np.outer(t[m, modes], np.conj(t[m, modes])) for m in range(t.shape[0])])
+    return b0, bms, g
+
+
+def general_input_norm(input_occupation, gram) -> complex:
+    """prod over input modes with n_j > 1 of perm(G[block_j, block_j]), block_j
+    the particles of mode j (probabilities.py:402-426), in double-double."""
+    g = np.asarray(gram, dtype=np.complex128)
+    norm = 1.0 + 0j
+    start = 0
+    for nj in input_occupation:
+        nj = int(nj)
+        if nj > 1:
+            mask = np.zeros(len(g), dtype=int)
+            mask[start:start + nj] = 1
+            norm *= perm_dd(g, mask, mask)
+        start += nj
+    return norm
+
+
+def lossy_pd_probabilities_dd(occupation_numbers, transmission_matrix, input_occupation,
+                              particle_overlap, conjugate_loss: bool = False):
+    """get_lossy_partially_distinguishable_detection_probabilities
+    (probabilities.py:244-384) with the subset sum in double-double: returns
+    (probabilities, error scale) per row of ``occupation_numbers``; the
+    scale is the absolute-value sum divided by |input norm|."""
+    occ = np.asarray(occupation_numbers, dtype=int)
+    if occ.ndim == 1:
+        occ = occ[None, :]
+    inp = np.asarray(input_occupation, dtype=int)
+    n = int(inp.sum())
+    probs = np.zeros(len(occ))
+    scale = np.zeros(len(occ))
+    detected = occ.sum(axis=1)
+    if n == 0:
+        probs[detected == 0] = 1.0
+        return probs, scale
+    valid = detected <= n
+    if not valid.any():
+        return probs, scale
+    b0, bms, g = lossy_pd_matrices(transmission_matrix, inp, particle_overlap, conjugate_loss)
+    if np.isscalar(particle_overlap):
+        norm = complex(uniform_input_norm(inp, float(particle_overlap)))
+    else:
+        norm = general_input_norm(inp, g)
+    coef, asum = perm_poly_dd(b0, bms, occ[valid])
+    probs[valid] = np.real(coef / norm)
+    scale[valid] = asum / abs(norm)
+    return probs, scale
 
 
 def gray_states(limits, offset: int = 0, count: Optional[int] = None) -> np.ndarray:

@@ -15,6 +15,8 @@ output_probabilities(U, s, outputs)  boson-sampling probabilities of many output
 permanent_laplace_batch(ms, rs, cs)  many permanent_laplace problems in one launch
 generate_samples / generate_lossy_samples  Clifford & Clifford sampling, shots in lockstep
 tensor_permanent / detection_probabilities  uniformly partially distinguishable photons
+lossy_detection_probabilities        any Gram matrix, lossy T (matrix-polynomial permanent)
+generate_lossy_and_partially_distinguishable_samples  sampling from those probabilities
 """
 
 from ._math.permanent import permanent, permanent_laplace, permanent_laplace_batch
@@ -22,9 +24,11 @@ from .engine import PermanentBatch, permanent_batch, permanent_ex
 from .grad import (PermanentBatchFunction, PermanentFunction, permanent_grad, permanent_grad_batch,
                    torch_permanent, torch_permanent_batch)
 from .probabilities import output_amplitudes, output_probabilities
-from .distinguishability import (detection_probabilities, detection_probability, tensor_permanent,
-                                 tensor_permanent_batch)
-from .sampling import generate_lossy_samples, generate_samples
+from .distinguishability import (detection_probabilities, detection_probability, general_input_norm,
+                                 lossy_detection_probabilities, perm_poly_coefficients,
+                                 tensor_permanent, tensor_permanent_batch)
+from .sampling import (fock_space_basis, generate_lossy_and_partially_distinguishable_samples,
+                       generate_lossy_samples, generate_samples, postselected_fock_basis)
 
 __all__ = [
     "permanent", "permanent_laplace", "permanent_ex", "PermanentBatch", "permanent_batch",
@@ -32,6 +36,8 @@ __all__ = [
     "output_amplitudes", "output_probabilities", "permanent_laplace_batch",
     "generate_samples", "generate_lossy_samples", "tensor_permanent", "tensor_permanent_batch",
     "detection_probability", "detection_probabilities", "torch_permanent_batch",
-    "PermanentBatchFunction",
+    "PermanentBatchFunction", "perm_poly_coefficients", "general_input_norm",
+    "lossy_detection_probabilities", "generate_lossy_and_partially_distinguishable_samples",
+    "fock_space_basis", "postselected_fock_basis",
 ]
 __version__ = "0.1.0"

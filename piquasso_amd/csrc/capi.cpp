@@ -19,6 +19,7 @@
 
 #include "perm_kernels.hpp"
 #include "plan.hpp"
+#include "poly_plan.hpp"
 #include "pqp_types.hpp"
 
 namespace {
@@ -634,6 +635,27 @@ int enqueue_all(pqp_batch *b, int mode, hipEvent_t ev_begin, hipEvent_t ev_end) 
 }
 }  // namespace
 
+namespace {
+
+// Device buffers of the matrix-polynomial permanent entry point, kept across
+// calls (grown on demand), with their own stream and lock.
+struct PolyState {
+  std::mutex mu;
+  bool init = false;
+  hipStream_t stream = nullptr;
+  DevBuf arena, scratch, acc, out;
+  HostBuf stage;
+};
+
+PolyState &poly_state() {
+  static PolyState *s = new PolyState();  // never destroyed: outlives atexit HIP teardown
+  return *s;
+}
+
+size_t align256(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
+
+}  // namespace
+
 extern "C" {
 
 const char *pqp_version(void) { return "pqp-permanent 0.1.0 (gfx950)"; }
@@ -829,6 +851,118 @@ int pqp_tensor_permanent_c128(const double *matrix, int64_t n, int64_t ld, doubl
                               double *out, double *abs_sum) {
   const double *m[1] = {matrix};
   return pqp_tensor_permanent_batch_c128(1, m, &n, &ld, &overlap, out, abs_sum);
+}
+
+int pqp_perm_poly_coefficients_c128(int64_t n, int64_t n_vars, const double *b0, const double *bm,
+                                    int64_t n_out, const int32_t *exponents, double *out,
+                                    int64_t *info) {
+  if (n_out < 0 || (n_out > 0 && !out) || (n >= 1 && !b0) || (n_vars > 0 && n >= 1 && !bm))
+    return fail(PQP_ERR_INVALID, "invalid matrix polynomial arguments");
+  PolyPlan p;
+  std::string err;
+  int st = poly_plan_build(n, n_vars, n_out, exponents, p, err);
+  if (st != PQP_OK) return fail(st, err);
+  for (int64_t q = 0; q < 2 * n_out; ++q) out[q] = 0.0;
+  const int64_t n_keep = static_cast<int64_t>(p.out_index.size());
+  const int64_t n_subsets = (int64_t{1} << p.n) - 1;
+  if (info) {
+    info[0] = p.n_mono;
+    info[1] = static_cast<int64_t>(p.links.size());
+    info[2] = 0;
+    info[3] = 0;
+    info[4] = p.flops_per_subset;
+    info[5] = n_subsets;
+  }
+  if (n_keep == 0) return PQP_OK;
+  if ((st = require_device()) != PQP_OK) return st;
+  const DeviceState &dev = device_state();
+  const int cus = std::max(dev.cus, 1);
+  const int nn = p.n, M = p.M;
+  const size_t rs_bytes = static_cast<size_t>(nn) * (M + 1) * sizeof(double2);
+  const size_t poly_bytes = static_cast<size_t>(p.n_mono) * sizeof(double2);
+  // the polynomial stays in LDS up to 64 KiB per workgroup (two or more
+  // workgroups per CU), else in a global-memory slice per workgroup
+  if (rs_bytes > 49152)
+    return fail(PQP_ERR_RANGE, "n x (variables + 1) = " + std::to_string(nn * (M + 1)) +
+                                   " row sums exceed the 3072 kept in LDS");
+  const bool lds = rs_bytes + poly_bytes <= 65536;
+  const size_t lds_bytes = lds ? rs_bytes + poly_bytes : rs_bytes;
+  int64_t per_cu = lds ? std::min<int64_t>(8, std::max<int64_t>(1, 163840 / static_cast<int64_t>(lds_bytes))) : 4;
+  int64_t n_wg = std::min<int64_t>(n_subsets, cus * per_cu);
+  // bound the per-workgroup slices: polynomials <= 8 GiB, accumulators <= 4 GiB
+  if (!lds) n_wg = std::min<int64_t>(n_wg, std::max<int64_t>(1, (int64_t{8} << 30) / static_cast<int64_t>(poly_bytes)));
+  n_wg = std::min<int64_t>(n_wg, std::max<int64_t>(1, (int64_t{4} << 30) / (n_keep * 16)));
+  if (!lds && poly_bytes > (size_t{8} << 30))
+    return fail(PQP_ERR_RANGE, "polynomial of " + std::to_string(p.n_mono) + " monomials is too large");
+  if (info) {
+    info[2] = n_wg;
+    info[3] = lds ? 1 : 0;
+  }
+  // arena: B_0, the kept B_m, link_ptr, links, lvl_off, out_mono
+  const size_t mat = static_cast<size_t>(nn) * nn * sizeof(double2);
+  size_t off = 0;
+  const size_t o_b0 = off;
+  off = align256(off + mat);
+  const size_t o_bm = off;
+  off = align256(off + mat * static_cast<size_t>(M));
+  const size_t o_lp = off;
+  off = align256(off + p.link_ptr.size() * sizeof(int32_t));
+  const size_t o_ln = off;
+  off = align256(off + std::max<size_t>(p.links.size(), 1) * sizeof(int32_t));
+  const size_t o_lv = off;
+  off = align256(off + p.lvl_off.size() * sizeof(int32_t));
+  const size_t o_om = off;
+  off = align256(off + p.out_mono.size() * sizeof(int32_t));
+  const size_t arena_bytes = off;
+  PolyState &S = poly_state();
+  std::lock_guard<std::mutex> lock(S.mu);
+  if (!S.init) {
+    PQP_HIP(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
+    S.init = true;
+  }
+  if ((st = S.arena.ensure(arena_bytes)) != PQP_OK) return st;
+  if ((st = S.stage.ensure(std::max(arena_bytes, static_cast<size_t>(n_keep) * 2 * sizeof(double)))) != PQP_OK)
+    return st;
+  if ((st = S.acc.ensure(static_cast<size_t>(n_wg) * static_cast<size_t>(n_keep) * sizeof(double2))) != PQP_OK)
+    return st;
+  if ((st = S.out.ensure(static_cast<size_t>(n_keep) * 2 * sizeof(double))) != PQP_OK) return st;
+  if (!lds && (st = S.scratch.ensure(static_cast<size_t>(n_wg) * poly_bytes)) != PQP_OK) return st;
+  char *h = static_cast<char *>(S.stage.p);
+  std::memcpy(h + o_b0, b0, mat);
+  for (int v = 0; v < M; ++v)
+    std::memcpy(h + o_bm + mat * v, bm + static_cast<size_t>(p.vars[v]) * nn * nn * 2, mat);
+  std::memcpy(h + o_lp, p.link_ptr.data(), p.link_ptr.size() * sizeof(int32_t));
+  if (!p.links.empty()) std::memcpy(h + o_ln, p.links.data(), p.links.size() * sizeof(int32_t));
+  std::memcpy(h + o_lv, p.lvl_off.data(), p.lvl_off.size() * sizeof(int32_t));
+  std::memcpy(h + o_om, p.out_mono.data(), p.out_mono.size() * sizeof(int32_t));
+  PQP_HIP(hipMemcpyAsync(S.arena.p, S.stage.p, arena_bytes, hipMemcpyHostToDevice, S.stream));
+  const char *d = static_cast<const char *>(S.arena.p);
+  PolyArgs a;
+  a.b0 = reinterpret_cast<const double2 *>(d + o_b0);
+  a.bm = reinterpret_cast<const double2 *>(d + o_bm);
+  a.link_ptr = reinterpret_cast<const int32_t *>(d + o_lp);
+  a.links = reinterpret_cast<const int32_t *>(d + o_ln);
+  a.lvl_off = reinterpret_cast<const int32_t *>(d + o_lv);
+  a.out_mono = reinterpret_cast<const int32_t *>(d + o_om);
+  a.scratch = lds ? nullptr : static_cast<double2 *>(S.scratch.p);
+  a.acc = static_cast<double2 *>(S.acc.p);
+  a.out = static_cast<double *>(S.out.p);
+  a.n = nn;
+  a.M = M;
+  a.L = p.L;
+  a.n_mono = static_cast<int32_t>(p.n_mono);
+  a.n_out = static_cast<int32_t>(n_keep);
+  a.n_wg = static_cast<int32_t>(n_wg);
+  PQP_HIP(launch_perm_poly(a, lds, lds_bytes, S.stream));
+  PQP_HIP(hipMemcpyAsync(S.stage.p, S.out.p, static_cast<size_t>(n_keep) * 2 * sizeof(double),
+                         hipMemcpyDeviceToHost, S.stream));
+  PQP_HIP(hipStreamSynchronize(S.stream));
+  const double *r = static_cast<const double *>(S.stage.p);
+  for (int64_t q = 0; q < n_keep; ++q) {
+    out[2 * p.out_index[q]] = r[2 * q];
+    out[2 * p.out_index[q] + 1] = r[2 * q + 1];
+  }
+  return PQP_OK;
 }
 
 int pqp_grad_batch_c128(int64_t count, const double *const *matrices, const int64_t *m_r,

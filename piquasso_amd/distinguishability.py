@@ -12,6 +12,17 @@ one lane per sign vector, a Gray walk over the other per wave,
 double-double accumulation, and only the real part (the two orders of a
 pair give conjugate terms, so the pair sum is real).
 
+The general case — any Gram matrix of the particles' internal states and a
+lossy (non-unitary) transmission matrix — is
+``get_lossy_partially_distinguishable_detection_probabilities``
+(probabilities.py:244-384): the coefficients of perm(B_loss + sum_m x_m B_m)
+by Ryser's formula over particle subsets, each subset's product of linear
+row factors kept as a truncated polynomial.  Here that subset loop runs in
+``perm_poly.hip`` (``pqp_perm_poly_coefficients_c128``), one workgroup per
+subset, with the polynomial kept on the monomials that divide a requested
+output pattern (the reference keeps the whole box up to the largest
+occupation per mode; the coefficients read out are the same).
+
 Deliberate difference: the reference returns the tensor permanent and the
 probability in a complex dtype whose imaginary part is zero up to rounding;
 these functions return real floats.
@@ -28,7 +39,9 @@ import numpy as np
 from . import _capi
 
 __all__ = ["tensor_permanent", "tensor_permanent_batch", "detection_probability",
-           "detection_probabilities", "uniform_input_norm"]
+           "detection_probabilities", "uniform_input_norm", "general_input_norm",
+           "perm_poly_coefficients", "lossy_detection_probabilities",
+           "get_lossy_partially_distinguishable_detection_probabilities"]
 
 MAX_PARTICLES = 20
 
@@ -129,3 +142,138 @@ def detection_probability(occupation_number, interferometer, input_occupation,
     (probabilities.py:156-190) for one output occupation."""
     return float(detection_probabilities([occupation_number], interferometer, input_occupation,
                                          uniform_particle_overlap)[0])
+
+
+def perm_poly_coefficients(b0, bms, exponents, with_info: bool = False):
+    """Coefficients [x^e] perm(B_0 + sum_m x_m B_m) for every row e of
+    ``exponents`` (shape (count, len(bms))), complex128 array, from
+    ``pqp_perm_poly_coefficients_c128``.  Rows with total degree above n give
+    0.  With ``with_info`` also a dict with the monomial count, links,
+    workgroups, whether the polynomial was kept in LDS, the algorithmic FP64
+    flops per subset and the number of subsets."""
+    b0 = np.ascontiguousarray(np.asarray(b0, dtype=np.complex128))
+    n = b0.shape[0]
+    if b0.ndim != 2 or b0.shape != (n, n) or n == 0:
+        raise ValueError(f"B_0 must be a non-empty square matrix, got shape {b0.shape}")
+    bm = np.ascontiguousarray(np.asarray(bms, dtype=np.complex128))
+    if bm.size == 0:
+        bm = np.zeros((0, n, n), dtype=np.complex128)
+    if bm.ndim != 3 or bm.shape[1:] != (n, n):
+        raise ValueError(f"B_m must be {n} x {n} matrices, got shape {bm.shape}")
+    e = np.ascontiguousarray(np.atleast_2d(np.asarray(exponents, dtype=np.int32)))
+    if e.ndim != 2 or e.shape[1] != bm.shape[0]:
+        raise ValueError("one exponent per variable in every row")
+    out = np.zeros(2 * max(len(e), 1))
+    info = np.zeros(6, dtype=np.int64)
+    dp = ctypes.POINTER(ctypes.c_double)
+    st = _capi.lib().pqp_perm_poly_coefficients_c128(
+        n, bm.shape[0], b0.view(np.float64).ctypes.data_as(dp),
+        bm.view(np.float64).ctypes.data_as(dp) if bm.size else None, len(e),
+        e.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), out.ctypes.data_as(dp),
+        info.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    _capi.check(st)
+    coef = out[: 2 * len(e)].view(np.complex128).copy()
+    if not with_info:
+        return coef
+    keys = ("monomials", "links", "workgroups", "lds", "flops_per_subset", "subsets")
+    return coef, dict(zip(keys, (int(v) for v in info)))
+
+
+def general_input_norm(input_occupation, particle_overlap_matrix) -> complex:
+    """prod over input modes holding more than one particle of
+    perm(G; mask, mask), mask selecting that mode's particles
+    (``_general_input_norm``, probabilities.py:402-426), each permanent from
+    the GPU."""
+    from ._math.permanent import permanent
+
+    g = np.asarray(particle_overlap_matrix, dtype=np.complex128)
+    norm = 1.0 + 0.0j
+    start = 0
+    for nj in np.asarray(input_occupation, dtype=np.int64).reshape(-1):
+        nj = int(nj)
+        if nj > 1:
+            mask = np.zeros(len(g), dtype=np.int32)
+            mask[start:start + nj] = 1
+            norm *= complex(permanent(g, mask, mask))
+        start += nj
+    return norm
+
+
+def _lossy_matrices(t, input_occupation, particle_overlap, conjugate_loss_kernel=False):
+    """B_loss = G o (I - T^dag T) and B_m = G o (t_m t_m^dag) on the
+    first-quantized input modes, t_m = T[m, input_modes]
+    (probabilities.py:304-338); with ``conjugate_loss_kernel`` the loss
+    kernel is conj(I - T^dag T)."""
+    modes = np.repeat(np.arange(len(input_occupation)), input_occupation)
+    n = len(modes)
+    if np.isscalar(particle_overlap):
+        x = float(particle_overlap)
+        g = x * np.ones((n, n), dtype=np.complex128) + (1.0 - x) * np.eye(n)
+    else:
+        g = np.asarray(particle_overlap, dtype=np.complex128)
+        if g.shape != (n, n):
+            raise ValueError(f"the particle overlap matrix must be {n} x {n}, got {g.shape}")
+    kernel = (np.eye(t.shape[1]) - t.conj().T @ t)[np.ix_(modes, modes)]
+    if conjugate_loss_kernel:
+        kernel = np.conj(kernel)
+    tm = t[:, modes]
+    bms = g[None, :, :] * (tm[:, :, None] * np.conj(tm)[:, None, :])
+    return g * kernel, bms, g
+
+
+def lossy_detection_probabilities(occupation_numbers, transmission_matrix, input_occupation,
+                                  particle_overlap, conjugate_loss_kernel: bool = False
+                                  ) -> np.ndarray:
+    """Detection probabilities of the output occupations (rows of
+    ``occupation_numbers``, one entry per output mode) for the input
+    occupation through the transmission matrix T (lossy or not), with a
+    scalar uniform particle overlap or a Gram matrix of the particles'
+    internal states (first-quantized order): the coefficients of
+    perm(B_loss + sum_m x_m B_m) divided by the input norm
+    (probabilities.py:244-384).  Outputs with more photons than the input
+    get 0; with no input photons the vacuum has probability 1.
+
+    By default B_loss = G o (I - T^dag T) as in the reference.  Then
+    sum_m B_m = G o (T^dag T)^T, so when T^dag T is not real (losses after a
+    complex interferometer) the outputs with lost photons are off and the
+    table does not sum to 1; ``conjugate_loss_kernel=True`` uses
+    G o conj(I - T^dag T), which reproduces the probabilities of the unitary
+    dilation of T.  Both agree whenever T^dag T is real (unitary T, uniform
+    losses, losses before the interferometer)."""
+    t = np.asarray(transmission_matrix, dtype=np.complex128)
+    if t.ndim != 2:
+        raise ValueError("the transmission matrix must be 2-D")
+    inp = np.asarray(input_occupation, dtype=np.int64).reshape(-1)
+    outs = np.asarray(occupation_numbers, dtype=np.int64)
+    if outs.ndim == 1:
+        outs = outs[None, :]
+    if inp.shape[0] != t.shape[1] or outs.shape[1] != t.shape[0]:
+        raise ValueError("occupations must have one entry per input / output mode of T")
+    if (inp < 0).any() or (outs < 0).any():
+        raise ValueError("occupation numbers must be non-negative")
+    n = int(inp.sum())
+    probs = np.zeros(outs.shape[0])
+    detected = outs.sum(axis=1)
+    if n == 0:
+        probs[detected == 0] = 1.0
+        return probs
+    valid = detected <= n
+    if not valid.any():
+        return probs
+    b0, bms, g = _lossy_matrices(t, inp, particle_overlap, conjugate_loss_kernel)
+    if np.isscalar(particle_overlap):
+        norm = complex(uniform_input_norm(inp, float(particle_overlap)))
+    else:
+        norm = general_input_norm(inp, g)
+    coef = perm_poly_coefficients(b0, bms, outs[valid])
+    probs[valid] = np.real(coef / norm)
+    return probs
+
+
+def get_lossy_partially_distinguishable_detection_probabilities(
+        occupation_numbers, transmission_matrix, input_occupation, particle_overlap,
+        connector=None) -> np.ndarray:
+    """The reference's name and argument order
+    (probabilities.py:244-250); ``connector`` is accepted and ignored."""
+    return lossy_detection_probabilities(occupation_numbers, transmission_matrix,
+                                         input_occupation, particle_overlap)

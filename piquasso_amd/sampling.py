@@ -48,7 +48,8 @@ import numpy as np
 from . import _capi
 
 __all__ = ["generate_samples", "generate_lossy_samples", "expanded_lossy_matrix",
-           "SampleGenerationError", "last_run"]
+           "generate_lossy_and_partially_distinguishable_samples", "fock_space_basis",
+           "postselected_fock_basis", "SampleGenerationError", "last_run"]
 
 # Counters of the most recent generate_samples call in this process: rounds,
 # column-minor problems, the seconds spent in the batched minors calls
@@ -437,3 +438,81 @@ def generate_lossy_samples(input, shots: int, interferometer,
     out = generate_samples(full, shots, big, postselect_data=postselect_data, config=config,
                            seed_sequence=seed_sequence)
     return [s[:d] for s in out]
+
+
+def _compositions(boxes: int, particles: int):
+    """All ways to put ``particles`` into ``boxes``, anti-lexicographic (the
+    first box fullest first), as piquasso's ``partitions``
+    (piquasso/_math/combinatorics.py:62-108) lists them."""
+    if boxes == 0:
+        return [()] if particles == 0 else []
+    if boxes == 1:
+        return [(particles,)]
+    out = []
+    for first in range(particles, -1, -1):
+        out.extend((first,) + rest for rest in _compositions(boxes - 1, particles - first))
+    return out
+
+
+def fock_space_basis(d: int, cutoff: int) -> np.ndarray:
+    """Occupations of d modes with 0..cutoff-1 particles, by particle number,
+    each block anti-lexicographic (``get_fock_space_basis``,
+    piquasso/_math/fock.py:65-80)."""
+    rows = [c for k in range(max(int(cutoff), 0)) for c in _compositions(int(d), k)]
+    return np.array(rows, dtype=np.int64).reshape(len(rows), int(d))
+
+
+def postselected_fock_basis(d: int, cutoff: int, postselected_modes, postselected_photons):
+    """``get_postselected_fock_basis`` (piquasso/_math/fock.py:83-113): the
+    basis of the other modes with cutoff reduced by the postselected photons,
+    the postselected modes filled in."""
+    modes = list(postselected_modes)
+    photons = np.asarray(postselected_photons, dtype=np.int64)
+    active = fock_space_basis(d - len(modes), int(cutoff) - int(photons.sum()))
+    if not modes:
+        return active
+    full = np.zeros((len(active), d), dtype=np.int64)
+    full[:, np.delete(np.arange(d), modes)] = active
+    full[:, modes] = photons
+    return full
+
+
+def generate_lossy_and_partially_distinguishable_samples(
+        input, shots: int, interferometer, postselect_data: Sequence = ((), (), 1000),
+        particle_overlap_matrix=None, config=None, *, rng=None,
+        seed_sequence: Optional[int] = None, conjugate_loss_kernel: bool = False):
+    """Samples of piquasso's ``generate_lossy_and_partially_distinguishable_samples``
+    (piquasso/_simulators/passive/sampling.py:658-708): the detection
+    probabilities of every output with at most n photons (postselected modes
+    fixed) from ONE matrix-polynomial permanent call
+    (``lossy_detection_probabilities``), normalised, and ``shots`` indices
+    drawn with ``rng.choice(count, size=shots, p=...)``.  The generator is
+    ``rng``, else ``config.rng`` (piquasso's Config), else
+    ``default_rng(seed_sequence)``.  ``particle_overlap_matrix`` is the Gram
+    matrix of the particles (first-quantized order) or a scalar overlap;
+    ``conjugate_loss_kernel`` is passed on to the probabilities.
+
+    Deliberate difference: probabilities that come out negative only through
+    rounding (above -1e-12 times the largest) are set to 0 before the draw;
+    numpy's ``choice`` would reject them."""
+    from .distinguishability import lossy_detection_probabilities
+
+    occ = np.asarray(input, dtype=np.int64).reshape(-1)
+    t = np.asarray(interferometer, dtype=np.complex128)
+    if shots < 0:
+        raise ValueError("shots must be non-negative")
+    if rng is None:
+        rng = getattr(config, "rng", None) if config is not None else None
+    if rng is None:
+        rng = np.random.default_rng(seed_sequence)
+    ps_modes, ps_photons = list(postselect_data[0]), postselect_data[1]
+    d = len(occ)
+    basis = postselected_fock_basis(d, int(occ.sum()) + 1, ps_modes, ps_photons)
+    trimmed = np.delete(basis, ps_modes, axis=1) if ps_modes else basis
+    overlap = 1.0 if particle_overlap_matrix is None else particle_overlap_matrix
+    p = lossy_detection_probabilities(basis, t, occ, overlap, conjugate_loss_kernel)
+    top = float(np.max(p)) if len(p) else 0.0
+    p = np.where((p < 0) & (p >= -1e-12 * top), 0.0, p)
+    p = p / np.sum(p)
+    idx = rng.choice(len(basis), size=shots, p=p)
+    return [tuple(int(v) for v in row) for row in trimmed[idx]]
