@@ -59,6 +59,11 @@ TENSOR_N = 18
 GRADIENT_CPU_SECONDS = 5.0
 PROB_CPU_SECONDS = 4.0
 TENSOR_CPU_N = 9
+# lossy partially distinguishable tables: (modes, photons); the first is
+# also run by the numpy port of the reference loop on LOSSY_PD_CPU_SUBSETS
+# of its 63 particle subsets
+LOSSY_PD_CONFIGS = ((7, 6), (10, 10))
+LOSSY_PD_CPU_SUBSETS = 8
 
 
 def _load_batched():
@@ -139,7 +144,24 @@ def cpu_baseline_child(seconds: float) -> None:
                       "threads": int(os.environ.get("OMP_THREAD_LIMIT", "0") or 0),
                       "singles_us": singles, "sampling": _cpu_sampling(SAMPLING_CPU_SECONDS),
                       "tensor": _cpu_tensor(), "gradient": _cpu_gradient(GRADIENT_CPU_SECONDS),
-                      "laplace_us": _cpu_laplace(), "probabilities": _cpu_probabilities(PROB_CPU_SECONDS)}))
+                      "laplace_us": _cpu_laplace(), "probabilities": _cpu_probabilities(PROB_CPU_SECONDS),
+                      "lossy_pd": _cpu_lossy_pd()}))
+
+
+def _cpu_lossy_pd():
+    """The reference's lossy partially distinguishable probability loop
+    (numpy restatement, oracle/lossy_pd_port.py) on the first subsets of the
+    7-mode, 6-photon table: the box polynomial of (6 + 1)^7 coefficients per
+    subset, so the full table costs (63 / subsets) times the sample."""
+    from oracle.lossy_pd_port import lossy_pd_port
+    from piquasso_amd import workloads as W
+
+    d, n = LOSSY_PD_CONFIGS[0]
+    t, inp, g, outs = W.lossy_pd(d, n)
+    t0 = time.perf_counter()
+    lossy_pd_port(outs, t, inp, g, subsets=LOSSY_PD_CPU_SUBSETS)
+    return {"modes": d, "photons": n, "subsets": LOSSY_PD_CPU_SUBSETS, "of": (1 << n) - 1,
+            "seconds": time.perf_counter() - t0}
 
 
 def _cpu_probabilities(seconds: float):
@@ -271,6 +293,7 @@ def run_cpu_baseline(seconds: float):
         "gradient": rec.get("gradient"),
         "laplace_us": rec.get("laplace_us"),
         "probabilities": rec.get("probabilities"),
+        "lossy_pd": rec.get("lossy_pd"),
     }
 
 
@@ -389,6 +412,63 @@ def _probabilities_leg(pa, W, cpu):
     return rec
 
 
+def _lossy_pd_leg(pa, W, cpu):
+    """Lossy, partially distinguishable probability tables (every output with
+    at most n photons; random Gram matrix; losses before a Haar unitary):
+    the matrix-polynomial permanent over all particle subsets in one call,
+    timed around the whole call (upload, kernels, download).  `ms` is the
+    median over new Gram matrices for the same output set (the monomial plan
+    is reused), `ms_first_call` a call that plans the closure afresh.
+    FLOP: the kernel's algorithmic count per subset (complex multiply 6, FMA
+    8 per link) times the subsets."""
+    from piquasso_amd import distinguishability as D
+
+    recs = {}
+    for d, n in LOSSY_PD_CONFIGS:
+        t, inp, g, outs = W.lossy_pd(d, n)
+        b0, bms, _ = D._lossy_matrices(t, inp, g)
+        # a first call with another output set, so the timed first call plans afresh
+        D.perm_poly_coefficients(b0, bms, outs[:1])
+        t0 = time.perf_counter()
+        pa.lossy_detection_probabilities(outs, t, inp, g)
+        cold = time.perf_counter() - t0
+        _, info = D.perm_poly_coefficients(b0, bms, outs, with_info=True)
+        ts = []
+        for k in range(3):
+            # new overlaps each time: the same output set, so the plan is reused
+            gk = W.lossy_pd(d, n, seed=20 + k)[2]
+            t0 = time.perf_counter()
+            p = pa.lossy_detection_probabilities(outs, t, inp, gk)
+            ts.append(time.perf_counter() - t0)
+        sec = float(np.median(ts))
+        fl = float(info["flops_per_subset"]) * info["subsets"]
+        rec = {
+            "outputs": len(outs),
+            "ms": sec * 1e3,
+            "ms_first_call": cold * 1e3,
+            "tflops": fl / sec / 1e12,
+            "flops": fl,
+            "monomials": info["monomials"],
+            "polynomial_in": "LDS" if info["lds"] else "global memory",
+            "workgroups": info["workgroups"],
+            "sum_minus_one": float(p.sum() - 1.0),
+            "what": f"lossy_detection_probabilities: all outputs with <= {n} photons in {d} modes, "
+                    f"{n} photons, random {n} x {n} Gram matrix",
+        }
+        c = (cpu or {}).get("lossy_pd")
+        if c and (c["modes"], c["photons"]) == (d, n):
+            full = c["seconds"] * c["of"] / c["subsets"]
+            rec["reference_cpu"] = {
+                "seconds_full_table_est": full,
+                "kind": "port",
+                "sample": f"numpy restatement of the reference's subset loop (box polynomial), "
+                          f"{c['subsets']} of {c['of']} subsets in {c['seconds']:.2f} s, one thread",
+            }
+            rec["speedup_vs_reference_cpu"] = full / sec
+        recs[f"modes{d}_photons{n}"] = rec
+    return recs
+
+
 def _sampling_leg(pa, W, cpu):
     """Clifford & Clifford sampling of 20 photons in 40 modes, 1024 shots in
     lockstep (20 rounds, each one batched column-minor launch over all
@@ -443,7 +523,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-child", type=float, default=None)
-    ap.add_argument("--profile", choices=["batch", "n32", "grad", "sampling", "tensor"], default=None,
+    ap.add_argument("--profile", choices=["batch", "n32", "grad", "sampling", "tensor", "lossy_pd"],
+                    default=None,
                     help="profiling runs: only launch this workload's kernels --steps times")
     ap.add_argument("--replay", choices=["eager", "graph"], default="eager",
                     help="profiling runs: direct launches or hipGraph replays")
@@ -476,6 +557,15 @@ def main():
             for _ in range(args.steps):
                 pa.tensor_permanent(a, 0.5)
             print(json.dumps({"profile": "tensor", "n": TENSOR_N}))
+            return 0
+        if args.profile == "lossy_pd":
+            from piquasso_amd import workloads as W
+
+            d, n = LOSSY_PD_CONFIGS[1]
+            t, inp, g, outs = W.lossy_pd(d, n)
+            for _ in range(args.steps):
+                pa.lossy_detection_probabilities(outs, t, inp, g)
+            print(json.dumps({"profile": "lossy_pd", "modes": d, "photons": n}))
             return 0
         if args.profile == "sampling":
             from piquasso_amd import workloads as W
@@ -594,6 +684,7 @@ def main():
     sampling = _sampling_leg(pa, W, cpu)
     tensor = _tensor_leg(pa, W, cpu)
     probabilities = _probabilities_leg(pa, W, cpu)
+    lossy_pd = _lossy_pd_leg(pa, W, cpu)
 
     # single permanents through the drop-in entry (host buffers in, result
     # out: planning, upload, sweep, download); the reference's CPU time for
@@ -670,6 +761,7 @@ def main():
         "gradient": grad,
         "sampling": sampling,
         "tensor_permanent": tensor,
+        "lossy_pd_probabilities": lossy_pd,
         "output_probabilities": probabilities,
         "single_permanents": singles,
         "roofline": {
@@ -687,7 +779,8 @@ def main():
             "units": stats["units"],
         },
         "cpu_baseline": ({k: v for k, v in cpu.items()
-                          if k not in ("sampling", "tensor", "gradient", "laplace_us", "probabilities")}
+                          if k not in ("sampling", "tensor", "gradient", "laplace_us", "probabilities",
+                                       "lossy_pd")}
                          if cpu else cpu),
     }
     print(json.dumps(line))

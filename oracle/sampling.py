@@ -155,6 +155,97 @@ def _distinguishable_outputs(interferometer, particles, rng, reject):
     return out
 
 
+def times_linear(poly, const, lin, out):
+    """(const + sum_a lin[a] x_a) * poly, truncated to poly's shape, as
+    multiply_by_linear_truncated (piquasso/_math/polynomial.py:18-38) writes
+    it: out = const * poly, then out += lin[a] * (poly shifted along axis a)
+    axis by axis.  With out = poly (the in-place call of sampling.py:545-548)
+    each shift reads the values already updated."""
+    out[...] = const * poly
+    for axis, c in enumerate(lin):
+        np.moveaxis(out, axis, 0)[1:] += c * np.moveaxis(poly, axis, 0)[:-1]
+    return out
+
+
+def _ps_probability(u, dist, modes, remaining):
+    """sampling.py:532-551 (in-place polynomial updates)."""
+    poly = np.zeros(tuple(np.asarray(remaining) + 1))
+    poly[(0,) * len(remaining)] = 1.0
+    for mode, mult in enumerate(dist):
+        p = np.abs(u[modes, mode]) ** 2
+        for _ in range(int(mult)):
+            times_linear(poly, 1.0 - p.sum(), p, out=poly)
+    return poly[tuple(remaining)]
+
+
+def _ps_table(u, dist, modes, bound):
+    """sampling.py:554-582."""
+    fq = _first_quantized(dist)
+    table = [np.zeros(tuple(np.asarray(bound) + 1)) for _ in range(len(fq) + 1)]
+    table[-1][(0,) * len(bound)] = 1.0
+    for k in range(len(fq) - 1, -1, -1):
+        p = np.abs(u[modes, fq[k]]) ** 2
+        times_linear(table[k + 1], 1.0 - p.sum(), p, out=table[k])
+    return table
+
+
+def _ps_conditioned(u, dist, modes, remaining, table, rng):
+    """sampling.py:585-655: each distinguishable particle to a free mode, to
+    loss, or to a postselected mode still short of photons."""
+    d = u.shape[0]
+    free = [m for m in range(d) if m not in set(modes)]
+    out = np.zeros(d, dtype=int)
+    left = np.array(remaining, dtype=int)
+    for k, mode in enumerate(_first_quantized(dist)):
+        nxt = table[k + 1]
+        fut = nxt[tuple(left)]
+        pf = np.abs(u[free, mode]) ** 2
+        pp = np.abs(u[list(modes), mode]) ** 2
+        ploss = 1.0 - pf.sum() - pp.sum()
+        wp = np.zeros(len(modes))
+        for a in range(len(modes)):
+            if left[a] > 0:
+                left[a] -= 1
+                wp[a] = pp[a] * nxt[tuple(left)]
+                left[a] += 1
+        w = np.concatenate([pf * fut, np.array([ploss * fut]), wp])
+        w /= np.sum(w)
+        i = rng.choice(len(w), p=w)
+        if i < len(free):
+            out[free[i]] += 1
+        elif i > len(free):
+            out[modes[i - len(free) - 1]] += 1
+            left[i - len(free) - 1] -= 1
+    return out
+
+
+def _one_sample_postselected_overlap(d, occ, u, rng, reject, laplace, modes, photons, max_trials,
+                                     overlap):
+    """sampling.py:422-529: split, one postselection attempt for the
+    indistinguishable particles, accept the distinguishable ones with their
+    (in-place evaluated) postselection probability, place them conditioned;
+    restart on failure."""
+    modes = list(modes)
+    photons = np.asarray(photons, dtype=int)
+    trials = 0
+    while True:
+        trials += 1
+        if trials > max_trials:
+            raise SamplingAborted("Too many trials during sample generation.")
+        indist, dist = _split_particles(occ, overlap, rng)
+        try:
+            ind = _one_sample_postselected(d, int(indist.sum()), _first_quantized(indist), u, rng,
+                                           reject, laplace, modes, photons, 1, track_needed=False)
+        except SamplingAborted:
+            continue
+        remaining = photons - ind[modes]
+        if rng.random() > _ps_probability(u, dist, modes, remaining):
+            continue
+        table = _ps_table(u, dist, modes, photons)
+        out = _ps_conditioned(u, dist, modes, remaining, table, rng) + ind
+        return np.delete(out, modes)
+
+
 def generate_samples(input_occupation, shots: int, interferometer, seed: int,
                      reject_condition: Optional[Callable[[], bool]] = None,
                      postselect: Sequence = ((), (), 1000),
@@ -162,7 +253,8 @@ def generate_samples(input_occupation, shots: int, interferometer, seed: int,
                      laplace=laplace_dd):
     """Samples of ``generate_samples`` (sampling.py:34-107) for the ideal,
     uniformly lossy (``reject_condition``), postselected and uniformly
-    partially distinguishable cases; one shot after another."""
+    partially distinguishable cases, and postselection with uniform partial
+    distinguishability; one shot after another."""
     occ = np.asarray(input_occupation, dtype=int)
     d = len(occ)
     n = int(occ.sum())
@@ -181,7 +273,11 @@ def generate_samples(input_occupation, shots: int, interferometer, seed: int,
                 s = _one_sample(d, n, pool, u, rng, reject, laplace)
         else:
             if len(ps_modes):
-                raise NotImplementedError("postselection with partial distinguishability")
+                s = _one_sample_postselected_overlap(d, occ, u, rng, reject, laplace, ps_modes,
+                                                     ps_photons, max_trials,
+                                                     float(uniform_particle_overlap))
+                samples.append(tuple(int(v) for v in s))
+                continue
             # sampling.py:328-368
             indist, dist = _split_particles(occ, uniform_particle_overlap, rng)
             s = _distinguishable_outputs(u, dist, rng, reject) + _one_sample(

@@ -645,6 +645,12 @@ struct PolyState {
   hipStream_t stream = nullptr;
   DevBuf arena, scratch, acc, out;
   HostBuf stage;
+  // the last plan and the exponent rows it was built for
+  bool have_plan = false;
+  bool tables_on_device = false;  // the plan's tables are in `arena`
+  int64_t key_n = 0, key_vars = 0, key_out = 0;
+  std::vector<int32_t> key_exps;
+  PolyPlan plan;
 };
 
 PolyState &poly_state() {
@@ -858,10 +864,27 @@ int pqp_perm_poly_coefficients_c128(int64_t n, int64_t n_vars, const double *b0,
                                     int64_t *info) {
   if (n_out < 0 || (n_out > 0 && !out) || (n >= 1 && !b0) || (n_vars > 0 && n >= 1 && !bm))
     return fail(PQP_ERR_INVALID, "invalid matrix polynomial arguments");
-  PolyPlan p;
-  std::string err;
-  int st = poly_plan_build(n, n_vars, n_out, exponents, p, err);
-  if (st != PQP_OK) return fail(st, err);
+  PolyState &S = poly_state();
+  std::lock_guard<std::mutex> lock(S.mu);
+  // the plan depends only on (n, the exponent rows): a call with the same
+  // rows as the previous one reuses it and the tables already on the device
+  const size_t n_ex = (n_out > 0 && n_vars > 0) ? static_cast<size_t>(n_out) * static_cast<size_t>(n_vars) : 0;
+  const bool hit = S.have_plan && S.key_n == n && S.key_vars == n_vars && S.key_out == n_out &&
+                   (n_ex == 0 || std::memcmp(S.key_exps.data(), exponents, n_ex * sizeof(int32_t)) == 0);
+  int st = PQP_OK;
+  if (!hit) {
+    S.have_plan = false;
+    S.tables_on_device = false;
+    std::string err;
+    st = poly_plan_build(n, n_vars, n_out, exponents, S.plan, err);
+    if (st != PQP_OK) return fail(st, err);
+    S.key_n = n;
+    S.key_vars = n_vars;
+    S.key_out = n_out;
+    S.key_exps.assign(exponents, exponents + n_ex);
+    S.have_plan = true;
+  }
+  const PolyPlan &p = S.plan;
   for (int64_t q = 0; q < 2 * n_out; ++q) out[q] = 0.0;
   const int64_t n_keep = static_cast<int64_t>(p.out_index.size());
   const int64_t n_subsets = (int64_t{1} << p.n) - 1;
@@ -887,7 +910,7 @@ int pqp_perm_poly_coefficients_c128(int64_t n, int64_t n_vars, const double *b0,
                                    " row sums exceed the 3072 kept in LDS");
   const bool lds = rs_bytes + poly_bytes <= 65536;
   const size_t lds_bytes = lds ? rs_bytes + poly_bytes : rs_bytes;
-  int64_t per_cu = lds ? std::min<int64_t>(8, std::max<int64_t>(1, 163840 / static_cast<int64_t>(lds_bytes))) : 4;
+  int64_t per_cu = lds ? std::min<int64_t>(8, std::max<int64_t>(1, 163840 / static_cast<int64_t>(lds_bytes))) : 1;
   int64_t n_wg = std::min<int64_t>(n_subsets, cus * per_cu);
   // bound the per-workgroup slices: polynomials <= 8 GiB, accumulators <= 4 GiB
   if (!lds) n_wg = std::min<int64_t>(n_wg, std::max<int64_t>(1, (int64_t{8} << 30) / static_cast<int64_t>(poly_bytes)));
@@ -898,13 +921,10 @@ int pqp_perm_poly_coefficients_c128(int64_t n, int64_t n_vars, const double *b0,
     info[2] = n_wg;
     info[3] = lds ? 1 : 0;
   }
-  // arena: B_0, the kept B_m, link_ptr, links, lvl_off, out_mono
+  // arena: the plan's tables (link_ptr, links, lvl_off, out_mono), then
+  // B_0 and the kept B_m (uploaded on every call)
   const size_t mat = static_cast<size_t>(nn) * nn * sizeof(double2);
   size_t off = 0;
-  const size_t o_b0 = off;
-  off = align256(off + mat);
-  const size_t o_bm = off;
-  off = align256(off + mat * static_cast<size_t>(M));
   const size_t o_lp = off;
   off = align256(off + p.link_ptr.size() * sizeof(int32_t));
   const size_t o_ln = off;
@@ -913,14 +933,18 @@ int pqp_perm_poly_coefficients_c128(int64_t n, int64_t n_vars, const double *b0,
   off = align256(off + p.lvl_off.size() * sizeof(int32_t));
   const size_t o_om = off;
   off = align256(off + p.out_mono.size() * sizeof(int32_t));
+  const size_t o_b0 = off;
+  off = align256(off + mat);
+  const size_t o_bm = off;
+  off = align256(off + mat * static_cast<size_t>(M));
   const size_t arena_bytes = off;
-  PolyState &S = poly_state();
-  std::lock_guard<std::mutex> lock(S.mu);
   if (!S.init) {
     PQP_HIP(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
     S.init = true;
   }
+  const void *arena_before = S.arena.p;
   if ((st = S.arena.ensure(arena_bytes)) != PQP_OK) return st;
+  if (S.arena.p != arena_before) S.tables_on_device = false;
   if ((st = S.stage.ensure(std::max(arena_bytes, static_cast<size_t>(n_keep) * 2 * sizeof(double)))) != PQP_OK)
     return st;
   if ((st = S.acc.ensure(static_cast<size_t>(n_wg) * static_cast<size_t>(n_keep) * sizeof(double2))) != PQP_OK)
@@ -928,14 +952,18 @@ int pqp_perm_poly_coefficients_c128(int64_t n, int64_t n_vars, const double *b0,
   if ((st = S.out.ensure(static_cast<size_t>(n_keep) * 2 * sizeof(double))) != PQP_OK) return st;
   if (!lds && (st = S.scratch.ensure(static_cast<size_t>(n_wg) * poly_bytes)) != PQP_OK) return st;
   char *h = static_cast<char *>(S.stage.p);
+  const size_t first = S.tables_on_device ? o_b0 : 0;
+  if (!S.tables_on_device) {
+    std::memcpy(h + o_lp, p.link_ptr.data(), p.link_ptr.size() * sizeof(int32_t));
+    if (!p.links.empty()) std::memcpy(h + o_ln, p.links.data(), p.links.size() * sizeof(int32_t));
+    std::memcpy(h + o_lv, p.lvl_off.data(), p.lvl_off.size() * sizeof(int32_t));
+    std::memcpy(h + o_om, p.out_mono.data(), p.out_mono.size() * sizeof(int32_t));
+  }
   std::memcpy(h + o_b0, b0, mat);
   for (int v = 0; v < M; ++v)
     std::memcpy(h + o_bm + mat * v, bm + static_cast<size_t>(p.vars[v]) * nn * nn * 2, mat);
-  std::memcpy(h + o_lp, p.link_ptr.data(), p.link_ptr.size() * sizeof(int32_t));
-  if (!p.links.empty()) std::memcpy(h + o_ln, p.links.data(), p.links.size() * sizeof(int32_t));
-  std::memcpy(h + o_lv, p.lvl_off.data(), p.lvl_off.size() * sizeof(int32_t));
-  std::memcpy(h + o_om, p.out_mono.data(), p.out_mono.size() * sizeof(int32_t));
-  PQP_HIP(hipMemcpyAsync(S.arena.p, S.stage.p, arena_bytes, hipMemcpyHostToDevice, S.stream));
+  PQP_HIP(hipMemcpyAsync(static_cast<char *>(S.arena.p) + first, h + first, arena_bytes - first,
+                         hipMemcpyHostToDevice, S.stream));
   const char *d = static_cast<const char *>(S.arena.p);
   PolyArgs a;
   a.b0 = reinterpret_cast<const double2 *>(d + o_b0);
@@ -957,6 +985,8 @@ int pqp_perm_poly_coefficients_c128(int64_t n, int64_t n_vars, const double *b0,
   PQP_HIP(hipMemcpyAsync(S.stage.p, S.out.p, static_cast<size_t>(n_keep) * 2 * sizeof(double),
                          hipMemcpyDeviceToHost, S.stream));
   PQP_HIP(hipStreamSynchronize(S.stream));
+  // the tables stay in the device arena for the next call with these rows
+  S.tables_on_device = true;
   const double *r = static_cast<const double *>(S.stage.p);
   for (int64_t q = 0; q < n_keep; ++q) {
     out[2 * p.out_index[q]] = r[2 * q];

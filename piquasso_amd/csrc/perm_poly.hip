@@ -21,8 +21,10 @@
 // from the top down with a workgroup barrier between levels, in place.
 //
 // One workgroup per subset at a time (subsets strided over the grid).  The
-// polynomial lives in LDS when it fits (kLds), else in a per-workgroup
-// global-memory slice.  Each workgroup adds its subsets' signed coefficients
+// polynomial lives in LDS when it fits (kLds; 256 threads, several
+// workgroups per CU), else in a per-workgroup global-memory slice (1024
+// threads, about one workgroup per CU, so the slices and accumulators stay
+// few).  Each workgroup adds its subsets' signed coefficients
 // into its own FP64 accumulator; perm_poly_reduce_kernel then sums the
 // workgroups' accumulators in double-double in workgroup order, so results
 // do not depend on timing.
@@ -47,8 +49,8 @@ __device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 c) {
                       __builtin_fma(a.x, b.y, __builtin_fma(a.y, b.x, c.y)));
 }
 
-template <bool kLds>
-__global__ __launch_bounds__(kPolyThreads) void perm_poly_kernel(PolyArgs a) {
+template <bool kLds, int kThreads>
+__global__ __launch_bounds__(kThreads) void perm_poly_kernel(PolyArgs a) {
   extern __shared__ double2 smem[];
   const int n = a.n, M = a.M, L = a.L;
   const int tid = static_cast<int>(threadIdx.x);
@@ -64,7 +66,7 @@ __global__ __launch_bounds__(kPolyThreads) void perm_poly_kernel(PolyArgs a) {
   for (uint32_t s = 1u + static_cast<uint32_t>(wg); s <= n_subsets; s += static_cast<uint32_t>(a.n_wg)) {
     // the previous subset's accumulation has read the polynomial
     __syncthreads();
-    for (int t = tid; t < n * (M + 1); t += kPolyThreads) {
+    for (int t = tid; t < n * (M + 1); t += kThreads) {
       const int i = t / (M + 1), m = t - i * (M + 1);
       const double2 *B = (m == M) ? a.b0 : a.bm + static_cast<int64_t>(m) * n * n;
       double2 v = make_double2(0.0, 0.0);
@@ -86,7 +88,7 @@ __global__ __launch_bounds__(kPolyThreads) void perm_poly_kernel(PolyArgs a) {
         // degree i + 1 is new in this row: it has no old value to scale
         const bool fresh = (D == i + 1);
         const int lo = a.lvl_off[D], hi = a.lvl_off[D + 1];
-        for (int m_idx = lo + tid; m_idx < hi; m_idx += kPolyThreads) {
+        for (int m_idx = lo + tid; m_idx < hi; m_idx += kThreads) {
           double2 v = fresh ? make_double2(0.0, 0.0) : cmul(c, poly[m_idx]);
           const int k1 = a.link_ptr[m_idx + 1];
           for (int k = a.link_ptr[m_idx]; k < k1; ++k) {
@@ -99,7 +101,7 @@ __global__ __launch_bounds__(kPolyThreads) void perm_poly_kernel(PolyArgs a) {
       }
     }
     const double sign = ((n - __builtin_popcount(s)) & 1) ? -1.0 : 1.0;
-    for (int p = tid; p < a.n_out; p += kPolyThreads) {
+    for (int p = tid; p < a.n_out; p += kThreads) {
       const double2 v = poly[a.out_mono[p]];
       if (first) {
         acc[p] = make_double2(sign * v.x, sign * v.y);
@@ -131,9 +133,11 @@ __global__ __launch_bounds__(256) void perm_poly_reduce_kernel(PolyArgs a) {
 
 hipError_t launch_perm_poly(const PolyArgs &a, bool lds, size_t lds_bytes, hipStream_t stream) {
   if (lds)
-    hipLaunchKernelGGL(perm_poly_kernel<true>, dim3(a.n_wg), dim3(kPolyThreads), lds_bytes, stream, a);
+    hipLaunchKernelGGL((perm_poly_kernel<true, kPolyThreads>), dim3(a.n_wg), dim3(kPolyThreads),
+                       lds_bytes, stream, a);
   else
-    hipLaunchKernelGGL(perm_poly_kernel<false>, dim3(a.n_wg), dim3(kPolyThreads), lds_bytes, stream, a);
+    hipLaunchKernelGGL((perm_poly_kernel<false, kPolyThreadsGlobal>), dim3(a.n_wg),
+                       dim3(kPolyThreadsGlobal), lds_bytes, stream, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(perm_poly_reduce_kernel, dim3((a.n_out + 255) / 256), dim3(256), 0, stream, a);

@@ -14,7 +14,8 @@ namespace pqp {
 constexpr int kPolyMaxN = 24;         // 2^24 - 1 column subsets at most
 constexpr int kPolyMaxVars = 255;     // variable index in the low 8 bits of a link
 constexpr int kPolyMaxMono = 1 << 23; // monomial index in the upper 23 bits of a link
-constexpr int kPolyThreads = 256;
+constexpr int kPolyThreads = 256;          // polynomial in LDS
+constexpr int kPolyThreadsGlobal = 1024;   // polynomial in global memory
 
 struct PolyArgs {
   const double2 *b0;       // n x n, row-major

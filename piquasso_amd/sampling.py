@@ -30,9 +30,12 @@ reference's order, so the batching cannot change them:
 
 Supported: ideal and uniformly lossy sampling (sampling.py:208-236),
 postselection (:239-325), uniform partial distinguishability
-(:328-419) and non-uniform losses through the 2d-mode unitary dilation
-(:110-146,756-784).  Not supported: postselection together with partial
-distinguishability (:422-529), which raises ``NotImplementedError``.
+(:328-419), postselection together with uniform partial distinguishability
+(:422-655; shots one after another, restarts being data-dependent),
+non-uniform losses through the 2d-mode unitary dilation (:110-146,756-784)
+and, for any Gram matrix of the particles with losses,
+``generate_lossy_and_partially_distinguishable_samples`` (:658-708) from the
+GPU's probability table.
 """
 
 from __future__ import annotations
@@ -137,7 +140,8 @@ class _Lockstep:
     order; ``postselect`` = (modes, photons, max_trials) or None.
     """
 
-    def __init__(self, minors: _MinorsBatch, rngs, pools, reject, postselect=None):
+    def __init__(self, minors: _MinorsBatch, rngs, pools, reject, postselect=None,
+                 track_needed: bool = True):
         self.minors = minors
         self.u_t = np.ascontiguousarray(minors.u.T)
         d = minors.d
@@ -152,6 +156,7 @@ class _Lockstep:
             self.pool0[s, : len(p)] = p
         self.reject = reject
         self.ps = postselect
+        self.track_needed = track_needed
         self.cur = np.zeros((S, d), dtype=np.int32)
         self.smp = np.zeros((S, d), dtype=np.int32)
         self.pool = self.pool0.copy()
@@ -251,8 +256,9 @@ class _Lockstep:
                 if hit.any():
                     over[hit] = (self.left[live[hit]] < 0).any(axis=1)
                 # too few photons left for the postselected modes (sampling.py:314-315)
-                short = ~over & (self.needed[live] > self.n[live] - self.step[live])
-                fail = live[over | short]
+                if self.track_needed:
+                    over |= self.needed[live] > self.n[live] - self.step[live]
+                fail = live[over]
                 if fail.size:
                     self._restart(fail)
 
@@ -332,9 +338,6 @@ def generate_samples(input, shots: int, interferometer,
     seed = _seed_of(config, seed_sequence)
     ps_modes, ps_photons, max_trials = postselect_data
     postselected = len(ps_modes) > 0
-    if postselected and uniform_particle_overlap is not None:
-        raise NotImplementedError(
-            "postselection together with partial distinguishability is not supported")
     minors = _MinorsBatch(u)
     if shots == 0:
         return []
@@ -361,6 +364,13 @@ def _sample_shots(minors, occ, shots, seed, reject_condition, ps_modes, ps_photo
     n = int(occ.sum())
     postselected = len(ps_modes) > 0
     rngs = [np.random.default_rng(seed=seed + s) for s in range(shots)]
+
+    if postselected and uniform_particle_overlap is not None:
+        # restarts depend on the draws: the shots run one after another
+        return [_postselected_overlap_shot(minors, occ, rngs[s], reject_condition, ps_modes,
+                                           ps_photons, max_trials, float(uniform_particle_overlap),
+                                           engines)
+                for s in range(shots)]
 
     if postselected and reject_condition is not None:
         # the number of reject_condition calls depends on the restarts: keep
@@ -410,6 +420,116 @@ def _sample_shots(minors, occ, shots, seed, reject_condition, ps_modes, ps_photo
     if postselected:
         smp = np.delete(smp, list(ps_modes), axis=1)
     return [tuple(row) for row in smp.tolist()]
+
+
+def _times_linear(poly, const, lin, out):
+    """(const + sum_a lin[a] x_a) * poly truncated to poly's shape, written to
+    ``out`` axis by axis (piquasso/_math/polynomial.py:18-38).  When ``out``
+    is ``poly`` itself, every axis step reads the values the previous steps
+    left, which yields const * poly * prod_a (1 + lin[a] x_a); the reference
+    calls it that way for the postselection probability of the
+    distinguishable particles (sampling.py:532-551), and so does
+    :func:`_dist_postselection_probability`, to draw the same samples."""
+    out[...] = const * poly
+    for axis, c in enumerate(lin):
+        np.moveaxis(out, axis, 0)[1:] += c * np.moveaxis(poly, axis, 0)[:-1]
+    return out
+
+
+def _dist_postselection_probability(u, dist, ps_modes, remaining):
+    """sampling.py:532-551: the postselected modes' polynomial over the
+    distinguishable particles, evaluated in place as the reference does."""
+    poly = np.zeros(tuple(remaining + 1), dtype=np.float64)
+    poly[(0,) * len(remaining)] = 1.0
+    for mode, count in enumerate(dist):
+        p = np.abs(u[ps_modes, mode]) ** 2
+        for _ in range(int(count)):
+            _times_linear(poly, 1.0 - p.sum(), p, out=poly)
+    return poly[tuple(remaining)]
+
+
+def _dist_postselection_table(u, dist, ps_modes, bound):
+    """sampling.py:554-582: table[k] = the postselected modes' polynomial of
+    the distinguishable particles k, k+1, ... (first-quantized order)."""
+    fq = _first_quantized(dist)
+    table = [np.zeros(tuple(bound + 1), dtype=np.float64) for _ in range(len(fq) + 1)]
+    table[-1][(0,) * len(bound)] = 1.0
+    for k in range(len(fq) - 1, -1, -1):
+        p = np.abs(u[ps_modes, fq[k]]) ** 2
+        _times_linear(table[k + 1], 1.0 - p.sum(), p, out=table[k])
+    return table
+
+
+def _dist_conditioned_output(u, dist, ps_modes, remaining, table, rng):
+    """sampling.py:585-655: the distinguishable particles one by one, each to
+    a free mode, lost, or to a postselected mode that still needs photons,
+    weighted by the table of what the later particles can still supply."""
+    d = u.shape[0]
+    free = np.delete(np.arange(d), ps_modes)
+    out = np.zeros(d, dtype=np.int64)
+    left = np.array(remaining, dtype=np.int64)
+    for k, mode in enumerate(_first_quantized(dist)):
+        nxt = table[k + 1]
+        future = nxt[tuple(left)]
+        p_free = np.abs(u[free, mode]) ** 2
+        p_ps = np.abs(u[ps_modes, mode]) ** 2
+        p_loss = 1.0 - p_free.sum() - p_ps.sum()
+        w_ps = np.zeros(len(ps_modes), dtype=np.float64)
+        for a, pa in enumerate(p_ps):
+            if left[a] == 0:
+                continue
+            left[a] -= 1
+            w_ps[a] = pa * nxt[tuple(left)]
+            left[a] += 1
+        w = np.concatenate([p_free * future, np.array([p_loss * future], dtype=np.float64), w_ps])
+        w /= np.sum(w)
+        idx = rng.choice(len(w), p=w)
+        if idx < len(free):
+            out[free[idx]] += 1
+        elif idx > len(free):
+            a = idx - len(free) - 1
+            out[ps_modes[a]] += 1
+            left[a] -= 1
+    return out
+
+
+def _postselected_overlap_shot(minors, occ, rng, reject_condition, ps_modes, ps_photons,
+                               max_trials, overlap, engines):
+    """One shot of postselected sampling with uniform partial
+    distinguishability (sampling.py:422-529): split the particles, sample the
+    indistinguishable ones with one postselection attempt (no look-ahead on
+    the photons still needed), accept the distinguishable ones' chance to
+    fill the remaining postselected photons with one draw, then place them
+    conditioned on it; restart on any failure, at most ``max_trials`` times."""
+    u = minors.u
+    modes = np.asarray(ps_modes, dtype=np.int64)
+    photons = np.asarray(ps_photons, dtype=np.int64)
+    live_reject = None
+    if reject_condition is not None:
+        live_reject = lambda idx: np.array([bool(reject_condition()) for _ in idx])  # noqa: E731
+    trial = 0
+    while True:
+        trial += 1
+        if trial > max_trials:
+            raise SampleGenerationError(
+                "Too many trials during sample generation. The specified postselection "
+                "criteria may be very highly unlikely. Aborting. To increase the limit, "
+                "set Config.max_sample_generation_trials to a higher value. Current value: "
+                f"{max_trials}.")
+        indist, dist = _split_particles(occ, overlap, rng)
+        eng = _Lockstep(minors, [rng], [_first_quantized(indist)], live_reject,
+                        (modes, photons, 1), track_needed=False)
+        engines.append(eng)
+        try:
+            indist_out = eng.run()[0].astype(np.int64)
+        except SampleGenerationError:
+            continue
+        remaining = photons - indist_out[modes]
+        if rng.random() > _dist_postselection_probability(u, dist, modes, remaining):
+            continue
+        table = _dist_postselection_table(u, dist, modes, photons)
+        out = _dist_conditioned_output(u, dist, modes, remaining, table, rng) + indist_out
+        return tuple(int(v) for v in np.delete(out, modes))
 
 
 def expanded_lossy_matrix(interferometer) -> np.ndarray:

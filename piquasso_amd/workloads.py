@@ -224,3 +224,22 @@ def output_patterns(modes: int = 14, photons: int = 7, seed: int = 8):
                      for c in itertools.combinations_with_replacement(range(modes), photons)],
                     dtype=np.int64)
     return u, inp, outs
+
+
+def lossy_pd(modes: int, photons: int, seed: int = 9):
+    """(T, input occupation, Gram matrix, outputs) for a lossy, partially
+    distinguishable probability table: T = U diag(sqrt(eta)) (a Haar unitary
+    after mode losses eta ~ U[0.6, 1]), one photon in each of the first
+    `photons` modes, a random Gram matrix of unit vectors, and every output
+    with at most `photons` photons (piquasso's Fock basis order)."""
+    from .sampling import fock_space_basis
+
+    rng = np.random.default_rng(seed)
+    u = haar_unitary(modes, rng)
+    eta = rng.uniform(0.6, 1.0, size=modes)
+    t = u @ np.diag(np.sqrt(eta))
+    v = rng.standard_normal((photons, photons)) + 1j * rng.standard_normal((photons, photons))
+    v /= np.linalg.norm(v, axis=1)[:, None]
+    inp = np.zeros(modes, dtype=np.int64)
+    inp[:photons] = 1
+    return t, inp, v @ v.conj().T, fock_space_basis(modes, photons + 1)

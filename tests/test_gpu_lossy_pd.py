@@ -79,6 +79,24 @@ def test_poly_no_variables_and_repeats(gpu, oracle):
     assert np.array_equal(pa.perm_poly_coefficients(a, bm, e), pa.perm_poly_coefficients(a, bm, e))
 
 
+def test_plan_reuse_across_calls(gpu, oracle):
+    """Calls with the same exponent rows reuse the plan and the device
+    tables (only the matrices are uploaded); each must still match the
+    oracle for its own matrices, also after a call with other rows."""
+    import piquasso_amd as pa
+
+    rng = np.random.default_rng(21)
+    n, m = 5, 3
+    e1 = rng.integers(0, 3, size=(20, m)).astype(np.int32)
+    e2 = rng.integers(0, 2, size=(7, m)).astype(np.int32)
+    for k, e in enumerate([e1, e1, e2, e1, e1]):
+        b0 = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        bm = rng.standard_normal((m, n, n)) + 1j * rng.standard_normal((m, n, n))
+        got = pa.perm_poly_coefficients(b0, bm, e)
+        ref, scale = oracle.perm_poly_dd(b0, bm, e)
+        assert np.all(np.abs(got - ref) <= _tol(n, scale)), k
+
+
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 def test_lossy_probabilities_reference_closed_forms(gpu, case):
     import piquasso_amd as pa

@@ -100,7 +100,8 @@ def test_sampler_reproduces_reference_samples(gpu, case):
     assert S.result_order(got, case["seed"]) == [tuple(e) for e in case["expected"]]
 
 
-@pytest.mark.parametrize("variant", ["ideal", "uniform_loss", "postselect", "overlap", "lossy"])
+@pytest.mark.parametrize("variant", ["ideal", "uniform_loss", "postselect", "overlap", "lossy",
+                                     "postselect_overlap"])
 def test_sampler_matches_oracle_sampler(gpu, variant):
     import piquasso_amd as pa
     from oracle import sampling as S
@@ -115,9 +116,12 @@ def test_sampler_matches_oracle_sampler(gpu, variant):
         kw_o["reject_condition"] = lambda: g2.uniform() > 0.8
     if variant == "overlap":
         kw_p["uniform_particle_overlap"] = kw_o["uniform_particle_overlap"] = 0.7
-    if variant == "postselect":
+    if variant.startswith("postselect"):
         kw_p["postselect_data"] = ((0, 5), (1, 1), 1000)
         kw_o["postselect"] = ((0, 5), (1, 1), 1000)
+    if variant == "postselect_overlap":
+        kw_p["uniform_particle_overlap"] = kw_o["uniform_particle_overlap"] = 0.7
+        u = 0.9 * u  # lossy: the distinguishable particles' loss weight stays positive
     print(f"sampler vs oracle: {variant}, d={d} n={sum(occ)} shots={shots}", flush=True)
     if variant == "lossy":
         t = u @ np.diag(np.linspace(0.95, 0.5, d)) @ _haar(d, 18)

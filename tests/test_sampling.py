@@ -114,7 +114,8 @@ def _haar(d, seed):
 
 
 @pytest.mark.parametrize("variant", ["ideal", "uniform_loss", "postselect", "overlap",
-                                     "overlap_loss", "lossy", "postselect_loss"])
+                                     "overlap_loss", "lossy", "postselect_loss",
+                                     "postselect_overlap", "postselect_overlap_loss"])
 def test_lockstep_host_logic_matches_oracle(lockstep_on_oracle, variant):
     from oracle import sampling as S
 
@@ -128,11 +129,16 @@ def test_lockstep_host_logic_matches_oracle(lockstep_on_oracle, variant):
         g2 = np.random.default_rng(99)
         kw_p["reject_condition"] = lambda: g1.uniform() > 0.7
         kw_o["reject_condition"] = lambda: g2.uniform() > 0.7
-    if variant.startswith("overlap"):
+    if "overlap" in variant:
         kw_p["uniform_particle_overlap"] = kw_o["uniform_particle_overlap"] = 0.6
     if variant.startswith("postselect"):
         kw_p["postselect_data"] = ((1, 4), (1, 0), 500)
         kw_o["postselect"] = ((1, 4), (1, 0), 500)
+    if "postselect_overlap" in variant:
+        # the distinguishable particles' loss weight is 1 - sum |T|^2, which
+        # rounding can make slightly negative for a unitary (numpy's choice
+        # then raises, in the reference as here): a lossy T
+        u = 0.9 * u
     if variant == "lossy":
         t = u @ np.diag([0.95, 0.9, 0.8, 0.7, 0.6, 0.5]) @ _haar(d, 4)
         got = P.generate_lossy_samples(occ, shots, t, seed_sequence=seed)
@@ -158,9 +164,6 @@ def test_sampler_argument_errors(lockstep_on_oracle):
         P.generate_samples([1, -1, 0], 2, u, seed_sequence=0)
     with pytest.raises(ValueError):
         P.generate_samples([1, 0], 2, u, seed_sequence=0)
-    with pytest.raises(NotImplementedError):
-        P.generate_samples([1, 1, 0], 2, u, postselect_data=((0,), (1,), 10),
-                           uniform_particle_overlap=0.5, seed_sequence=0)
     assert P.generate_samples([1, 1, 0], 0, u, seed_sequence=0) == []
 
 
@@ -169,3 +172,39 @@ def test_postselection_trial_limit(lockstep_on_oracle):
     # identity interferometer: mode 0 can never receive a photon from mode 1
     with pytest.raises(P.SampleGenerationError):
         P.generate_samples([0, 1], 3, np.eye(2), postselect_data=((0,), (1,), 5), seed_sequence=1)
+
+
+def test_in_place_linear_factor_semantics():
+    """The reference multiplies the distinguishable particles' postselection
+    polynomial in place (sampling.py:545-548 with polynomial.py:18-38): each
+    axis step then reads values already scaled by the constant.  One particle
+    and one postselected mode: the coefficient of x is c * p (c = 1 - p), not
+    p; with two modes the cross term c * p0 * p1 appears."""
+    from oracle.sampling import times_linear
+
+    poly = np.array([1.0, 0.0, 0.0])
+    times_linear(poly, 0.7, [0.3], out=poly)
+    assert poly.tolist() == [0.7, 0.7 * 0.3, 0.0]
+    fresh = times_linear(np.array([1.0, 0.0, 0.0]), 0.7, [0.3], out=np.zeros(3))
+    assert fresh.tolist() == [0.7, 0.3, 0.0]
+    p2 = np.zeros((2, 2))
+    p2[0, 0] = 1.0
+    times_linear(p2, 0.5, [0.2, 0.3], out=p2)
+    assert np.allclose(p2, [[0.5, 0.5 * 0.3], [0.5 * 0.2, 0.5 * 0.2 * 0.3]])
+
+
+def test_postselected_overlap_reference_support(lockstep_on_oracle):
+    """The reference's checks for postselection with distinguishable photons
+    (tests/_simulators/passive/test_measurements.py:1242-1268): a balanced
+    beamsplitter, overlap 0.5, one photon postselected in mode 0 leaves
+    exactly one photon in mode 1; with uniform losses 0 or 1."""
+    P = lockstep_on_oracle
+    bs = np.array([[1, 1], [1, -1]], dtype=complex) / np.sqrt(2)
+    out = P.generate_samples([1, 1], 10, bs, postselect_data=((0,), (1,), 1000),
+                             uniform_particle_overlap=0.5, seed_sequence=3)
+    assert out == [(1,)] * 10
+    g = np.random.default_rng(4)
+    out = P.generate_samples([1, 1], 10, np.sqrt(0.8) * bs, postselect_data=((0,), (1,), 1000),
+                             uniform_particle_overlap=0.5, seed_sequence=3,
+                             reject_condition=lambda: g.uniform() > 0.8)
+    assert set(out) <= {(0,), (1,)}
