@@ -220,6 +220,17 @@ int pqp_perm_poly_coefficients_c128(int64_t n, int64_t n_vars, const double *b0,
                                     int64_t *info);
 
 /*
+ * Planning only (no device involved): the monomial closure
+ * pqp_perm_poly_coefficients_c128 would use for these exponent rows, for
+ * inspection by tests of the planner.  info (6 entries): monomials, links,
+ * variables in use, largest kept total degree, algorithmic FP64 flops per
+ * subset, exponent rows kept (total degree <= n).  No counterpart in the
+ * reference.
+ */
+int pqp_perm_poly_plan_info(int64_t n, int64_t n_vars, int64_t n_out, const int32_t *exponents,
+                            int64_t *info);
+
+/*
  * Gradients of `count` permanents: grads[k] (m_r[k] x m_c[k] complex,
  * row-major, interleaved) = d perm(A_k; r_k, c_k) / d (A_k)_ij
  * = r_i c_j perm(A_k; r_k - e_i, c_k - e_j) (0 where r_i or c_j is 0), the

@@ -86,6 +86,9 @@ SIGNATURES = {
         _c.c_int,
         [_c.c_int64, _c.POINTER(_dp), _i64p, _i64p, _dp, _dp, _dp],
     ),
+    "pqp_perm_poly_plan_info": (
+        _c.c_int, [_c.c_int64, _c.c_int64, _c.c_int64, _i32p, _i64p],
+    ),
     "pqp_perm_poly_coefficients_c128": (
         _c.c_int,
         [_c.c_int64, _c.c_int64, _dp, _dp, _c.c_int64, _i32p, _dp, _i64p],

@@ -859,6 +859,22 @@ int pqp_tensor_permanent_c128(const double *matrix, int64_t n, int64_t ld, doubl
   return pqp_tensor_permanent_batch_c128(1, m, &n, &ld, &overlap, out, abs_sum);
 }
 
+int pqp_perm_poly_plan_info(int64_t n, int64_t n_vars, int64_t n_out, const int32_t *exponents,
+                            int64_t *info) {
+  if (!info) return fail(PQP_ERR_INVALID, "info is null");
+  PolyPlan p;
+  std::string err;
+  int st = poly_plan_build(n, n_vars, n_out, exponents, p, err);
+  if (st != PQP_OK) return fail(st, err);
+  info[0] = p.n_mono;
+  info[1] = static_cast<int64_t>(p.links.size());
+  info[2] = p.M;
+  info[3] = p.L;
+  info[4] = p.flops_per_subset;
+  info[5] = static_cast<int64_t>(p.out_index.size());
+  return PQP_OK;
+}
+
 int pqp_perm_poly_coefficients_c128(int64_t n, int64_t n_vars, const double *b0, const double *bm,
                                     int64_t n_out, const int32_t *exponents, double *out,
                                     int64_t *info) {

@@ -317,3 +317,83 @@ def test_sampler_host_logic(monkeypatch, oracle):
     p = oracle_probs(basis, t, inp, g)
     idx = np.random.default_rng(7).choice(len(basis), size=50, p=p / p.sum())
     assert got == [tuple(r) for r in np.delete(basis, [2], axis=1)[idx].tolist()]
+
+
+# ------------------------------------------------------------------ planner --
+
+def _plan_info(n, exps):
+    import ctypes
+
+    from piquasso_amd import _capi
+
+    e = np.ascontiguousarray(np.atleast_2d(np.asarray(exps, dtype=np.int32)))
+    info = np.zeros(6, dtype=np.int64)
+    st = _capi.lib().pqp_perm_poly_plan_info(
+        n, e.shape[1], e.shape[0], e.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+        info.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    _capi.check(st)
+    return dict(zip(("monomials", "links", "vars", "degree", "flops", "kept"), info.tolist()))
+
+
+def _closure(n, exps):
+    """Every monomial dividing a requested row of total degree <= n, on the
+    variables some kept row uses."""
+    keep = [tuple(e) for e in exps if sum(e) <= n]
+    used = [v for v in range(len(exps[0])) if any(e[v] for e in keep)]
+    seen = set()
+    for e in keep:
+        for a in itertools.product(*[range(e[v] + 1) for v in used]):
+            seen.add(a)
+    return seen, used, keep
+
+
+def _flops(n, mono):
+    """Per subset: row i updates degrees 0..min(i + 1, L); a new top degree
+    costs 8 per link - 2, an old one 6 + 8 per link (complex multiply and
+    FMAs of the level recursion)."""
+    L = max((sum(a) for a in mono), default=0)
+    cnt = [0] * (L + 1)
+    links = [0] * (L + 1)
+    for a in mono:
+        cnt[sum(a)] += 1
+        links[sum(a)] += sum(1 for x in a if x > 0)
+    total = 0
+    for i in range(n):
+        for D in range(min(i + 1, L) + 1):
+            total += 8 * links[D] - 2 * cnt[D] if D == i + 1 else 6 * cnt[D] + 8 * links[D]
+    return total
+
+
+@pytest.mark.parametrize("case", ["packed", "simplex", "wide", "empty_vars", "over_degree"])
+def test_poly_planner_closure(case):
+    """The planner's monomial closure, link count and FLOP count against a
+    direct enumeration (packed 64-bit keys and the byte-row fallback)."""
+    rng = np.random.default_rng(len(case))
+    if case == "packed":
+        n, exps = 6, rng.integers(0, 3, size=(15, 5)).tolist()
+    elif case == "simplex":
+        n, exps = 5, [list(p) for p in itertools.product(range(6), repeat=4) if sum(p) <= 5]
+    elif case == "wide":  # 33 variables x 2 bits > 64: byte rows
+        n, exps = 3, [[0] * 33 for _ in range(30)]
+        for e in exps:
+            for _ in range(int(rng.integers(0, 3))):
+                e[int(rng.integers(0, 33))] += 1
+    elif case == "empty_vars":
+        n, exps = 4, [[0, 1, 0, 2], [1, 0, 0, 0], [0, 0, 0, 1]]
+    else:
+        n, exps = 3, [[2, 2], [1, 1], [0, 3], [4, 0]]
+    got = _plan_info(n, exps)
+    mono, used, keep = _closure(n, exps)
+    assert got["monomials"] == len(mono) and got["vars"] == len(used) and got["kept"] == len(keep)
+    assert got["links"] == sum(sum(1 for x in a if x > 0) for a in mono)
+    assert got["degree"] == max(sum(e) for e in keep)
+    assert got["flops"] == _flops(n, mono)
+
+
+def test_poly_planner_errors():
+    with pytest.raises(ValueError):
+        _plan_info(25, [[1]])
+    with pytest.raises(ValueError):
+        _plan_info(3, [[-1]])
+    with pytest.raises(ValueError):
+        _plan_info(0, [[0]])
