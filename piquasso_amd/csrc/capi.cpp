@@ -925,14 +925,22 @@ int pqp_perm_poly_coefficients_c128(int64_t n, int64_t n_vars, const double *b0,
     return fail(PQP_ERR_RANGE, "n x (variables + 1) = " + std::to_string(nn * (M + 1)) +
                                    " row sums exceed the 3072 kept in LDS");
   const bool lds = rs_bytes + poly_bytes <= 65536;
-  const size_t lds_bytes = lds ? rs_bytes + poly_bytes : rs_bytes;
-  int64_t per_cu = lds ? std::min<int64_t>(8, std::max<int64_t>(1, 163840 / static_cast<int64_t>(lds_bytes))) : 1;
-  int64_t n_wg = std::min<int64_t>(n_subsets, cus * per_cu);
-  // bound the per-workgroup slices: polynomials <= 8 GiB, accumulators <= 4 GiB
-  if (!lds) n_wg = std::min<int64_t>(n_wg, std::max<int64_t>(1, (int64_t{8} << 30) / static_cast<int64_t>(poly_bytes)));
-  n_wg = std::min<int64_t>(n_wg, std::max<int64_t>(1, (int64_t{4} << 30) / (n_keep * 16)));
   if (!lds && poly_bytes > (size_t{8} << 30))
     return fail(PQP_ERR_RANGE, "polynomial of " + std::to_string(p.n_mono) + " monomials is too large");
+  // in global memory, kPolyGroup subsets share a workgroup (coefficients
+  // interleaved) when their row sums fit in LDS and their slice in 8 GiB
+  const int group = (!lds && kPolyGroup * rs_bytes <= 49152 &&
+                     kPolyGroup * poly_bytes <= (size_t{8} << 30)) ? kPolyGroup : 1;
+  const size_t slice_bytes = poly_bytes * static_cast<size_t>(group);
+  const size_t lds_bytes = lds ? rs_bytes + poly_bytes : rs_bytes * static_cast<size_t>(group);
+  // global memory: two 1024-thread workgroups per CU (32 waves) hide the
+  // gather latency best (measured against one per CU and other groupings)
+  int64_t per_cu = lds ? std::min<int64_t>(8, std::max<int64_t>(1, 163840 / static_cast<int64_t>(lds_bytes))) : 2;
+  const int64_t n_groups = (n_subsets + group - 1) / group;
+  int64_t n_wg = std::min<int64_t>(n_groups, cus * per_cu);
+  // bound the per-workgroup slices: polynomials <= 8 GiB, accumulators <= 4 GiB
+  if (!lds) n_wg = std::min<int64_t>(n_wg, std::max<int64_t>(1, (int64_t{8} << 30) / static_cast<int64_t>(slice_bytes)));
+  n_wg = std::min<int64_t>(n_wg, std::max<int64_t>(1, (int64_t{4} << 30) / (n_keep * 16)));
   if (info) {
     info[2] = n_wg;
     info[3] = lds ? 1 : 0;
@@ -966,7 +974,7 @@ int pqp_perm_poly_coefficients_c128(int64_t n, int64_t n_vars, const double *b0,
   if ((st = S.acc.ensure(static_cast<size_t>(n_wg) * static_cast<size_t>(n_keep) * sizeof(double2))) != PQP_OK)
     return st;
   if ((st = S.out.ensure(static_cast<size_t>(n_keep) * 2 * sizeof(double))) != PQP_OK) return st;
-  if (!lds && (st = S.scratch.ensure(static_cast<size_t>(n_wg) * poly_bytes)) != PQP_OK) return st;
+  if (!lds && (st = S.scratch.ensure(static_cast<size_t>(n_wg) * slice_bytes)) != PQP_OK) return st;
   char *h = static_cast<char *>(S.stage.p);
   const size_t first = S.tables_on_device ? o_b0 : 0;
   if (!S.tables_on_device) {
@@ -997,7 +1005,7 @@ int pqp_perm_poly_coefficients_c128(int64_t n, int64_t n_vars, const double *b0,
   a.n_mono = static_cast<int32_t>(p.n_mono);
   a.n_out = static_cast<int32_t>(n_keep);
   a.n_wg = static_cast<int32_t>(n_wg);
-  PQP_HIP(launch_perm_poly(a, lds, lds_bytes, S.stream));
+  PQP_HIP(launch_perm_poly(a, lds, group, lds_bytes, S.stream));
   PQP_HIP(hipMemcpyAsync(S.stage.p, S.out.p, static_cast<size_t>(n_keep) * 2 * sizeof(double),
                          hipMemcpyDeviceToHost, S.stream));
   PQP_HIP(hipStreamSynchronize(S.stream));

@@ -68,9 +68,11 @@ hipError_t launch_tensor(int n, const TensorArgs &a, int first_unit, int n_units
 hipError_t launch_tensor_finalize(const TensorArgs &a, hipStream_t stream);
 
 // Matrix-polynomial permanent (perm_poly.hip): the subset sweep over
-// a.n_wg workgroups (the polynomial in LDS when `lds`, dynamic LDS of
+// a.n_wg workgroups (the polynomial in LDS when `lds`; else in global memory
+// with `group` (1 or kPolyGroup) subsets interleaved; dynamic LDS of
 // lds_bytes), then the reduction of the workgroups' accumulators.
-hipError_t launch_perm_poly(const PolyArgs &a, bool lds, size_t lds_bytes, hipStream_t stream);
+hipError_t launch_perm_poly(const PolyArgs &a, bool lds, int group, size_t lds_bytes,
+                            hipStream_t stream);
 
 hipError_t launch_fwd(int bucket, const FwdLaunch &l);
 hipError_t occupancy_fwd(int bucket, bool persistent, bool fp32, bool comp, int *blocks_per_cu);

@@ -16,6 +16,7 @@ constexpr int kPolyMaxVars = 255;     // variable index in the low 8 bits of a l
 constexpr int kPolyMaxMono = 1 << 23; // monomial index in the upper 23 bits of a link
 constexpr int kPolyThreads = 256;          // polynomial in LDS
 constexpr int kPolyThreadsGlobal = 1024;   // polynomial in global memory
+constexpr int kPolyGroup = 2;              // subsets interleaved per workgroup (global memory)
 
 struct PolyArgs {
   const double2 *b0;       // n x n, row-major
@@ -24,7 +25,7 @@ struct PolyArgs {
   const int32_t *links;    // (index of a - e_m) << 8 | m, for every m with a_m > 0
   const int32_t *lvl_off;  // L + 2: monomials of total degree D are [lvl_off[D], lvl_off[D + 1])
   const int32_t *out_mono; // n_out: monomial of each requested coefficient
-  double2 *scratch;        // global-memory polynomials, n_mono per workgroup (when not in LDS)
+  double2 *scratch;        // global-memory polynomials, n_mono x group per workgroup (when not in LDS)
   double2 *acc;            // n_wg x n_out: each workgroup's signed subset sum
   double *out;             // 2 per requested coefficient
   int32_t n, M, L, n_mono, n_out, n_wg;

@@ -160,7 +160,9 @@ def perm_poly_coefficients(b0, bms, exponents, with_info: bool = False):
         bm = np.zeros((0, n, n), dtype=np.complex128)
     if bm.ndim != 3 or bm.shape[1:] != (n, n):
         raise ValueError(f"B_m must be {n} x {n} matrices, got shape {bm.shape}")
-    e = np.ascontiguousarray(np.atleast_2d(np.asarray(exponents, dtype=np.int32)))
+    e = np.atleast_2d(np.asarray(exponents))
+    if e.dtype != np.int32 or not e.flags.c_contiguous:
+        e = np.ascontiguousarray(e, dtype=np.int32)
     if e.ndim != 2 or e.shape[1] != bm.shape[0]:
         raise ValueError("one exponent per variable in every row")
     out = np.zeros(2 * max(len(e), 1))
@@ -244,30 +246,24 @@ def lossy_detection_probabilities(occupation_numbers, transmission_matrix, input
     if t.ndim != 2:
         raise ValueError("the transmission matrix must be 2-D")
     inp = np.asarray(input_occupation, dtype=np.int64).reshape(-1)
-    outs = np.asarray(occupation_numbers, dtype=np.int64)
+    outs = np.asarray(occupation_numbers)
     if outs.ndim == 1:
         outs = outs[None, :]
     if inp.shape[0] != t.shape[1] or outs.shape[1] != t.shape[0]:
         raise ValueError("occupations must have one entry per input / output mode of T")
-    if (inp < 0).any() or (outs < 0).any():
+    if (inp < 0).any():
         raise ValueError("occupation numbers must be non-negative")
     n = int(inp.sum())
-    probs = np.zeros(outs.shape[0])
-    detected = outs.sum(axis=1)
     if n == 0:
-        probs[detected == 0] = 1.0
-        return probs
-    valid = detected <= n
-    if not valid.any():
-        return probs
+        return (outs.sum(axis=1) == 0).astype(np.float64)
     b0, bms, g = _lossy_matrices(t, inp, particle_overlap, conjugate_loss_kernel)
     if np.isscalar(particle_overlap):
         norm = complex(uniform_input_norm(inp, float(particle_overlap)))
     else:
         norm = general_input_norm(inp, g)
-    coef = perm_poly_coefficients(b0, bms, outs[valid])
-    probs[valid] = np.real(coef / norm)
-    return probs
+    # outputs with more photons than the input (and negative entries, which
+    # the planner rejects) are handled in the C planner: their coefficient is 0
+    return np.real(perm_poly_coefficients(b0, bms, outs) / norm)
 
 
 def get_lossy_partially_distinguishable_detection_probabilities(

@@ -576,10 +576,10 @@ def _compositions(boxes: int, particles: int):
 
 def fock_space_basis(d: int, cutoff: int) -> np.ndarray:
     """Occupations of d modes with 0..cutoff-1 particles, by particle number,
-    each block anti-lexicographic (``get_fock_space_basis``,
+    each block anti-lexicographic, int32 (``get_fock_space_basis``,
     piquasso/_math/fock.py:65-80)."""
     rows = [c for k in range(max(int(cutoff), 0)) for c in _compositions(int(d), k)]
-    return np.array(rows, dtype=np.int64).reshape(len(rows), int(d))
+    return np.array(rows, dtype=np.int32).reshape(len(rows), int(d))
 
 
 def postselected_fock_basis(d: int, cutoff: int, postselected_modes, postselected_photons):
@@ -591,7 +591,7 @@ def postselected_fock_basis(d: int, cutoff: int, postselected_modes, postselecte
     active = fock_space_basis(d - len(modes), int(cutoff) - int(photons.sum()))
     if not modes:
         return active
-    full = np.zeros((len(active), d), dtype=np.int64)
+    full = np.zeros((len(active), d), dtype=np.int32)
     full[:, np.delete(np.arange(d), modes)] = active
     full[:, modes] = photons
     return full
