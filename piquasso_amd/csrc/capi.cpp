@@ -913,6 +913,8 @@ int pqp_perm_poly_coefficients_c128(int64_t n, int64_t n_vars, const double *b0,
     info[5] = n_subsets;
   }
   if (n_keep == 0) return PQP_OK;
+  if (n_keep > INT32_MAX / 2)
+    return fail(PQP_ERR_RANGE, std::to_string(n_keep) + " requested coefficients exceed the kernel's index range");
   if ((st = require_device()) != PQP_OK) return st;
   const DeviceState &dev = device_state();
   const int cus = std::max(dev.cus, 1);
