@@ -12,6 +12,7 @@ permanent_grad / permanent_grad_batch d perm / d A (grad_perm, any shape)
 torch_permanent(A, rows, cols)       differentiable permanent (torch.autograd.Function)
 torch_permanent_batch(A, rows, cols) the same for a (B, m_r, m_c) stack, one engine call
 output_probabilities(U, s, outputs)  boson-sampling probabilities of many outputs in one batch
+particle_number_probabilities(...)   the same for a superposition of input Fock states
 permanent_laplace_batch(ms, rs, cs)  many permanent_laplace problems in one launch
 generate_samples / generate_lossy_samples  Clifford & Clifford sampling, shots in lockstep
 tensor_permanent / detection_probabilities  uniformly partially distinguishable photons
@@ -23,7 +24,8 @@ from ._math.permanent import permanent, permanent_laplace, permanent_laplace_bat
 from .engine import PermanentBatch, permanent_batch, permanent_ex
 from .grad import (PermanentBatchFunction, PermanentFunction, permanent_grad, permanent_grad_batch,
                    torch_permanent, torch_permanent_batch)
-from .probabilities import output_amplitudes, output_probabilities
+from .probabilities import (output_amplitudes, output_probabilities,
+                            particle_number_probabilities)
 from .distinguishability import (detection_probabilities, detection_probability, general_input_norm,
                                  lossy_detection_probabilities, perm_poly_coefficients,
                                  tensor_permanent, tensor_permanent_batch)
@@ -38,6 +40,6 @@ __all__ = [
     "detection_probability", "detection_probabilities", "torch_permanent_batch",
     "PermanentBatchFunction", "perm_poly_coefficients", "general_input_norm",
     "lossy_detection_probabilities", "generate_lossy_and_partially_distinguishable_samples",
-    "fock_space_basis", "postselected_fock_basis",
+    "fock_space_basis", "postselected_fock_basis", "particle_number_probabilities",
 ]
 __version__ = "0.1.0"

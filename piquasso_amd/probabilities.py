@@ -18,7 +18,8 @@ import numpy as np
 
 from .engine import PermanentBatch
 
-__all__ = ["output_amplitudes", "output_probabilities"]
+__all__ = ["output_amplitudes", "output_probabilities", "particle_number_probabilities",
+           "calculate_inner_product", "get_ideal_particle_number_probability"]
 
 
 def _norm(occupation) -> float:
@@ -61,3 +62,57 @@ def output_probabilities(unitary, input_state: Sequence[int], output_states,
                          precision: str = "fp64") -> np.ndarray:
     """Probabilities |<t| U |s>|^2 for every output pattern t."""
     return np.abs(output_amplitudes(unitary, input_state, output_states, precision)) ** 2
+
+
+def particle_number_probabilities(output_states, interferometer, occupation_numbers, coefficients,
+                                  precision: str = "fp64") -> np.ndarray:
+    """|sum_j c_j <t| U |s_j>|^2 for every output pattern t (rows of
+    ``output_states``) of the superposition sum_j c_j |s_j> of input Fock
+    states (rows of ``occupation_numbers``): the reference's
+    ``get_ideal_particle_number_probability`` (probabilities.py:26-54) for
+    many outputs, every (output, input) pair of equal photon number in one
+    batched launch."""
+    u = np.ascontiguousarray(np.asarray(interferometer, dtype=np.complex128))
+    if u.ndim != 2 or u.shape[0] != u.shape[1]:
+        raise ValueError(f"interferometer must be square, got shape {u.shape}")
+    m = u.shape[0]
+    ins = np.asarray(occupation_numbers, dtype=np.int64)
+    if ins.ndim == 1:
+        ins = ins[None, :]
+    outs = np.asarray(output_states, dtype=np.int64)
+    if outs.ndim == 1:
+        outs = outs[None, :]
+    coefs = np.asarray(coefficients, dtype=np.complex128).reshape(-1)
+    if ins.shape[1] != m or outs.shape[1] != m or coefs.shape[0] != ins.shape[0]:
+        raise ValueError("one coefficient per input state and one entry per mode in every state")
+    if (ins < 0).any() or (outs < 0).any():
+        raise ValueError("occupation numbers must be non-negative")
+    n_out, n_in = outs.sum(axis=1), ins.sum(axis=1)
+    pair_out, pair_in = np.nonzero(n_out[:, None] == n_in[None, :])
+    amps = np.zeros(outs.shape[0], dtype=np.complex128)
+    if pair_out.size:
+        rows = np.ascontiguousarray(outs[pair_out], dtype=np.int32)
+        cols = np.ascontiguousarray(ins[pair_in], dtype=np.int32)
+        with PermanentBatch([u] * pair_out.size, list(rows), list(cols), precision=precision) as b:
+            b.run("eager")
+            perms = b.results()
+        top = int(max(outs.max(initial=0), ins.max(initial=0)))
+        fact = np.array([math.factorial(k) for k in range(top + 1)], dtype=np.float64)
+        norm = np.sqrt(np.prod(fact[rows], axis=1) * np.prod(fact[cols], axis=1))
+        np.add.at(amps, pair_out, coefs[pair_in] * perms / norm)
+    return np.abs(amps) ** 2
+
+
+def calculate_inner_product(interferometer, input, output, connector=None) -> complex:
+    """<output| U |input> = perm(U; rows=output, cols=input) /
+    sqrt(prod output! prod input!) (piquasso/_simulators/passive/utils.py:131-138);
+    ``connector`` is accepted and ignored."""
+    return complex(output_amplitudes(interferometer, input, [output])[0])
+
+
+def get_ideal_particle_number_probability(occupation_number, interferometer, occupation_numbers,
+                                          coefficients, connector=None) -> float:
+    """The reference's name and arguments (probabilities.py:26-54) for one
+    output pattern; ``connector`` is accepted and ignored."""
+    return float(particle_number_probabilities([occupation_number], interferometer,
+                                               occupation_numbers, coefficients)[0])

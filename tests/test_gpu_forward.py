@@ -323,3 +323,40 @@ def test_batch_shared_matrix_marshalling(gpu):
     np.testing.assert_array_equal(shared, single)
     p = pa.output_probabilities(u, inp, outs)
     assert p.shape == (len(outs),) and np.all(p >= 0)
+
+
+def test_superposition_particle_number_probabilities(gpu, oracle):
+    """particle_number_probabilities: |sum_j c_j <t|U|s_j>|^2 for a
+    superposition of input Fock states (the reference's
+    get_ideal_particle_number_probability, probabilities.py:26-54), against
+    the double-double oracle, one input reducing to output_probabilities,
+    the identity interferometer (|c_j|^2 on each input pattern) and the
+    normalisation of a unitary."""
+    from piquasso_amd.probabilities import (calculate_inner_product,
+                                            get_ideal_particle_number_probability)
+
+    rng = np.random.default_rng(12)
+    u = W.haar_unitary(5, rng)
+    ins = np.array([[1, 0, 1, 1, 0], [0, 2, 0, 0, 1], [3, 0, 0, 0, 0], [1, 1, 0, 0, 0]])
+    c = rng.standard_normal(4) + 1j * rng.standard_normal(4)
+    c /= np.linalg.norm(c[:3])  # the three 3-photon inputs are normalised; the 2-photon one is extra
+    outs = np.array([t for t in itertools.product(range(4), repeat=5) if sum(t) in (2, 3)])
+    p = pa.particle_number_probabilities(outs, u, ins, c)
+    fact = [math.factorial(k) for k in range(4)]
+    for t, pt in list(zip(outs, p))[::9]:
+        amp = 0j
+        for s, cj in zip(ins, c):
+            if s.sum() == t.sum():
+                amp += cj * oracle.perm_dd(u, t, s) / math.sqrt(
+                    np.prod([fact[k] for k in t]) * np.prod([fact[k] for k in s]))
+        assert abs(pt - abs(amp) ** 2) < 1e-13, t
+    three = outs.sum(axis=1) == 3
+    assert abs(p[three].sum() - 1.0) < 1e-12
+    assert abs(p[~three].sum() - abs(c[3]) ** 2) < 1e-12
+    one = pa.particle_number_probabilities(outs, u, ins[:1], [1.0])
+    np.testing.assert_allclose(one, pa.output_probabilities(u, ins[0], outs), atol=1e-15)
+    pid = pa.particle_number_probabilities(ins, np.eye(5), ins, c)
+    np.testing.assert_allclose(pid, np.abs(c) ** 2, atol=1e-15)
+    assert abs(get_ideal_particle_number_probability(outs[7], u, ins, c) - p[7]) < 1e-15
+    amp = calculate_inner_product(u, ins[0], outs[40])
+    assert abs(amp - complex(pa.output_amplitudes(u, ins[0], [outs[40]])[0])) == 0
