@@ -158,7 +158,12 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     total_cost += item_cost[k];
   }
 
-  // Pass 2: tables.
+  // Pass 2: tables.  The per-item work lists live outside the loop and are
+  // cleared per item, so their storage is allocated once per plan.
+  std::vector<int64_t> colsel, colsrc, src_rows;
+  std::vector<int> colmult, order, inner, top, profile;
+  std::vector<Digit> digits;
+  std::vector<char> is_inner;
   for (int64_t k = 0; k < n_items; ++k) {
     const PermInput &in = inputs[k];
     ItemDesc d;
@@ -217,6 +222,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
       d.partial_base = partial_cursor;
       if (opt.minors) {
         d.flags = kMinors;
+        plan.info[k].colsrc.reserve(static_cast<size_t>(in.m_c));
         for (int64_t j = 0; j < in.m_c; ++j)
           if (in.cols[j] > 0) plan.info[k].colsrc.push_back(j);
         d.mc = static_cast<int32_t>(plan.info[k].colsrc.size());
@@ -234,7 +240,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     const int n = static_cast<int>(sr);
 
     // --- compaction: columns with c_j > 0, sorted by multiplicity (desc) ---
-    std::vector<int64_t> colsel;
+    colsel.clear();
     for (int64_t j = 0; j < in.m_c; ++j)
       if (in.cols[j] > 0) colsel.push_back(j);
     std::stable_sort(colsel.begin(), colsel.end(), [&](int64_t x, int64_t y) {
@@ -252,8 +258,8 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
       if (opt.expand_cols == 1) expand = n <= kMaxCols;
       if (opt.expand_cols == 2) expand = n <= kExpandAutoMax;
     }
-    std::vector<int64_t> colsrc;  // source column of each kernel column
-    std::vector<int> colmult;     // multiplicity of each kernel column
+    colsrc.clear();   // source column of each kernel column
+    colmult.clear();  // multiplicity of each kernel column
     for (int64_t j : colsel) {
       if (expand) {
         for (int q = 0; q < in.cols[j]; ++q) {
@@ -287,7 +293,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
         minelem = in.rows[i];
         i0 = i;
       }
-    std::vector<Digit> digits;
+    digits.clear();
     for (int64_t i = 0; i < in.m_r; ++i) {
       int rho = in.rows[i] - (i == i0 ? 1 : 0);
       if (rho > 0) digits.push_back(Digit{i, rho});
@@ -307,7 +313,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     }
 
     // --- inner/top split: inner digits = smallest radices first ---
-    std::vector<int> order(D);
+    order.resize(D);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
       return digits[x].rho < digits[y].rho;
@@ -352,10 +358,10 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     }
     const int L = best_L;
     const int Dt = D - L;
-    std::vector<int> inner(order.begin(), order.begin() + L);
-    std::vector<int> top;
+    inner.assign(order.begin(), order.begin() + L);
+    top.clear();
     {
-      std::vector<char> is_inner(D, 0);
+      is_inner.assign(D, 0);
       for (int x : inner) is_inner[x] = 1;
       for (int x = 0; x < D; ++x)
         if (!is_inner[x]) top.push_back(x);
@@ -396,7 +402,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
       }
     };
     // B rows: prepended row, inner digit rows, top digit rows
-    std::vector<int64_t> src_rows;
+    src_rows.clear();
     src_rows.push_back(i0);
     for (int x : inner) src_rows.push_back(digits[x].src_row);
     for (int x : top) src_rows.push_back(digits[x].src_row);
@@ -415,7 +421,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     for (int t = 0; t < L; ++t)
       at<double>(plan.arena, d.alpha_off)[t] = static_cast<double>(digits[inner[t]].rho);
     // shared inner step / coefficient tables
-    std::vector<int> profile;
+    profile.clear();
     for (int x : inner) profile.push_back(digits[x].rho);
     auto it = inner_cache.find(profile);
     if (it == inner_cache.end()) {
