@@ -25,10 +25,12 @@
 // threads, several workgroups per CU), else in a per-workgroup global-memory
 // slice (1024 threads, two workgroups per CU, so the slices and
 // accumulators stay few; K = 2 subsets interleaved coefficient by
-// coefficient, so a gather fetches 32 contiguous bytes for two lanes).  Each workgroup adds its subsets' signed coefficients
-// into its own FP64 accumulator; perm_poly_reduce_kernel then sums the
-// workgroups' accumulators in double-double in workgroup order, so results
-// do not depend on timing.
+// coefficient, so a gather fetches 32 contiguous bytes for two lanes; each
+// lane issues four links' loads and gathers before its first FMA waits).
+// Each workgroup adds its subsets' signed coefficients into its own FP64
+// accumulator; perm_poly_reduce_kernel then sums the workgroups'
+// accumulators in double-double in workgroup order, so results do not
+// depend on timing.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
