@@ -18,6 +18,7 @@ generate_samples / generate_lossy_samples  Clifford & Clifford sampling, shots i
 tensor_permanent / detection_probabilities  uniformly partially distinguishable photons
 lossy_detection_probabilities        any Gram matrix, lossy T (matrix-polynomial permanent)
 generate_lossy_and_partially_distinguishable_samples  sampling from those probabilities
+marginal_fock_probabilities / generate_marginal_samples  marginals from permanent-polynomial moments
 """
 
 from ._math.permanent import permanent, permanent_laplace, permanent_laplace_batch
@@ -31,6 +32,7 @@ from .distinguishability import (detection_probabilities, detection_probability,
                                  tensor_permanent, tensor_permanent_batch)
 from .sampling import (fock_space_basis, generate_lossy_and_partially_distinguishable_samples,
                        generate_lossy_samples, generate_samples, postselected_fock_basis)
+from .marginal import generate_marginal_samples, marginal_fock_probabilities
 
 __all__ = [
     "permanent", "permanent_laplace", "permanent_ex", "PermanentBatch", "permanent_batch",
@@ -41,5 +43,6 @@ __all__ = [
     "PermanentBatchFunction", "perm_poly_coefficients", "general_input_norm",
     "lossy_detection_probabilities", "generate_lossy_and_partially_distinguishable_samples",
     "fock_space_basis", "postselected_fock_basis", "particle_number_probabilities",
+    "marginal_fock_probabilities", "generate_marginal_samples",
 ]
 __version__ = "0.1.0"
