@@ -130,6 +130,79 @@ class _MinorsBatch:
         return out.view(np.complex128)
 
 
+_U32 = np.uint64(0xFFFFFFFF)
+
+
+class _Draws:
+    """The per-shot draws the sampler needs, ``Generator.integers(0, L,
+    dtype=np.int64)`` and ``Generator.random()``, evaluated for many shots at
+    once from blocks of each shot's raw PCG64 outputs (``random_raw``), by
+    numpy's own rules: ``random()`` is the top 53 bits of the next 64-bit
+    output times 2^-53; ``integers`` takes a 32-bit half through the bit
+    generator's cached-half state (the low half of a fresh output, then its
+    high half on the next 32-bit draw) and applies Lemire's
+    multiply-and-reject method, and draws nothing for L = 1.  The values and
+    their order equal the generator's own calls (tested draw for draw).  The
+    generators' states run ahead by the unused part of the blocks, so the
+    engine must be the generators' last user."""
+
+    def __init__(self, rngs, block: int):
+        S = len(rngs)
+        self.bgs = [r.bit_generator for r in rngs]
+        self.raw = np.empty((S, max(block, 1)), dtype=np.uint64)
+        self.has32 = np.zeros(S, dtype=bool)
+        self.cache = np.zeros(S, dtype=np.uint64)
+        for k, bg in enumerate(self.bgs):
+            st = bg.state
+            self.has32[k] = bool(st["has_uint32"])
+            self.cache[k] = np.uint64(st["uinteger"])
+            self.raw[k] = bg.random_raw(self.raw.shape[1])
+        self.ptr = np.zeros(S, dtype=np.int64)
+
+    def _next64(self, shots):
+        while (self.ptr[shots] >= self.raw.shape[1]).any():
+            more = np.stack([bg.random_raw(self.raw.shape[1]) for bg in self.bgs])
+            self.raw = np.concatenate([self.raw, more], axis=1)
+        v = self.raw[shots, self.ptr[shots]]
+        self.ptr[shots] += 1
+        return v
+
+    def _next32(self, shots):
+        out = np.empty(len(shots), dtype=np.uint64)
+        h = self.has32[shots]
+        out[h] = self.cache[shots[h]]
+        self.has32[shots[h]] = False
+        fresh = shots[~h]
+        if fresh.size:
+            v = self._next64(fresh)
+            out[~h] = v & _U32
+            self.cache[fresh] = v >> np.uint64(32)
+            self.has32[fresh] = True
+        return out
+
+    def doubles(self, shots) -> np.ndarray:
+        return (self._next64(shots) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+    def integers(self, shots, bound) -> np.ndarray:
+        L = np.asarray(bound, dtype=np.uint64)
+        res = np.zeros(len(shots), dtype=np.int64)
+        act = L > 1
+        if not act.any():
+            return res
+        sh, Le = shots[act], L[act]
+        m = self._next32(sh) * Le
+        left = m & _U32
+        thr = (np.uint64(1 << 32) - Le) % Le
+        rej = (left < Le) & (left < thr)
+        while rej.any():
+            i = np.flatnonzero(rej)
+            m[i] = self._next32(sh[i]) * Le[i]
+            left[i] = m[i] & _U32
+            rej[i] = left[i] < thr[i]
+        res[act] = (m >> np.uint64(32)).astype(np.int64)
+        return res
+
+
 class _Lockstep:
     """S shots of Clifford & Clifford sampling advanced one photon per round.
 
@@ -141,7 +214,7 @@ class _Lockstep:
     """
 
     def __init__(self, minors: _MinorsBatch, rngs, pools, reject, postselect=None,
-                 track_needed: bool = True):
+                 track_needed: bool = True, fast_draws: bool = False):
         self.minors = minors
         self.u_t = np.ascontiguousarray(minors.u.T)
         d = minors.d
@@ -157,6 +230,8 @@ class _Lockstep:
         self.reject = reject
         self.ps = postselect
         self.track_needed = track_needed
+        # vectorised draws when this engine is the generators' last user
+        self.draws = _Draws(rngs, int(self.n.max()) * 3 // 2 + 4) if fast_draws and S else None
         self.cur = np.zeros((S, d), dtype=np.int32)
         self.smp = np.zeros((S, d), dtype=np.int32)
         self.pool = self.pool0.copy()
@@ -213,9 +288,13 @@ class _Lockstep:
                 return self.smp
             # grow the input by a uniformly chosen pool particle (sampling.py:197-205)
             plen = self.plen[live]
-            ints = self._ints
-            pick = np.fromiter((ints[s](0, L, dtype=np.int64) for s, L in zip(live.tolist(), plen.tolist())),
-                               dtype=np.int64, count=live.size)
+            if self.draws is not None:
+                pick = self.draws.integers(live, plen)
+            else:
+                ints = self._ints
+                pick = np.fromiter((ints[s](0, L, dtype=np.int64)
+                                    for s, L in zip(live.tolist(), plen.tolist())),
+                                   dtype=np.int64, count=live.size)
             self.cur[live, self.pool[live, pick]] += 1  # one entry per shot: no repeated index
             # drop the picked particle, keeping the pool's order
             width = self.pool.shape[1]
@@ -237,8 +316,11 @@ class _Lockstep:
                 raise ValueError("probabilities contain NaN")
             cdf = np.cumsum(pmf / norm, axis=1)
             cdf /= cdf[:, -1:]
-            rand = self._rand
-            u = np.fromiter((rand[s]() for s in live.tolist()), dtype=np.float64, count=live.size)
+            if self.draws is not None:
+                u = self.draws.doubles(live)
+            else:
+                rand = self._rand
+                u = np.fromiter((rand[s]() for s in live.tolist()), dtype=np.float64, count=live.size)
             mode = (cdf <= u[:, None]).sum(axis=1)
             mode = np.minimum(mode, cdf.shape[1] - 1)
             self.smp[live, mode] += 1
@@ -379,7 +461,7 @@ def _sample_shots(minors, occ, shots, seed, reject_condition, ps_modes, ps_photo
         for s in range(shots):
             live_reject = lambda idx: np.array([bool(reject_condition()) for _ in idx])  # noqa: E731
             eng = _Lockstep(minors, [rngs[s]], [_first_quantized(occ)], live_reject,
-                            (ps_modes, ps_photons, max_trials))
+                            (ps_modes, ps_photons, max_trials), fast_draws=True)
             engines.append(eng)
             smp = eng.run()[0]
             out.append(tuple(int(v) for v in np.delete(smp, list(ps_modes))))
@@ -412,7 +494,7 @@ def _sample_shots(minors, occ, shots, seed, reject_condition, ps_modes, ps_photo
             dist_out[s] = _distinguishable(u, dist, rngs[s], lambda s=s: reject_one(s))
             pools.append(_first_quantized(indist))
     eng = _Lockstep(minors, rngs, pools, reject_many if decisions is not None else None,
-                    (ps_modes, ps_photons, max_trials) if postselected else None)
+                    (ps_modes, ps_photons, max_trials) if postselected else None, fast_draws=True)
     engines.append(eng)
     smp = eng.run().astype(np.int64)
     if dist_out is not None:

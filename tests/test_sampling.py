@@ -208,3 +208,33 @@ def test_postselected_overlap_reference_support(lockstep_on_oracle):
                              uniform_particle_overlap=0.5, seed_sequence=3,
                              reject_condition=lambda: g.uniform() > 0.8)
     assert set(out) <= {(0,), (1,)}
+
+
+def test_vectorised_draws_equal_generator_calls():
+    """_Draws gives, shot by shot, the values Generator.integers(0, L,
+    dtype=np.int64) and Generator.random() give, in any interleaving: L = 1
+    (no bits used), small L, and L near 2^31 and 3 * 2^30 (Lemire's
+    rejection taken often), from generators that already hold a cached
+    32-bit half."""
+    from piquasso_amd.sampling import _Draws
+
+    rs = np.random.default_rng(0)
+    for trial in range(60):
+        seeds = rs.integers(0, 2 ** 32, 6)
+        real = [np.random.default_rng(int(x)) for x in seeds]
+        emu = [np.random.default_rng(int(x)) for x in seeds]
+        for g1, g2 in zip(real, emu):
+            for _ in range(trial % 3):
+                g1.integers(0, 5, dtype=np.int64)
+                g2.integers(0, 5, dtype=np.int64)
+        draws = _Draws(emu, block=4)
+        shots = np.arange(6)
+        for _ in range(25):
+            if rs.random() < 0.5:
+                bounds = rs.choice([1, 2, 3, 17, 40, 2 ** 31 + 3, 3 * 2 ** 30], size=6)
+                want = [g.integers(0, int(b), dtype=np.int64) for g, b in zip(real, bounds)]
+                got = draws.integers(shots, bounds)
+            else:
+                want = [g.random() for g in real]
+                got = draws.doubles(shots)
+            assert list(got) == want
