@@ -24,9 +24,11 @@ int64_t binom64(int n, int k) {
   return r;
 }
 
-// Append `bytes` of zeroed space to the arena at 256-byte alignment.
+// Append `bytes` of zeroed space to the arena at 64-byte alignment: table
+// starts on scalar-cache lines (no kernel load needs more; 256 bytes doubled
+// the arena of a batch of small permanents).
 int64_t arena_alloc(std::vector<char> &arena, int64_t bytes) {
-  int64_t off = align_up(static_cast<int64_t>(arena.size()), 256);
+  int64_t off = align_up(static_cast<int64_t>(arena.size()), 64);
   arena.resize(static_cast<size_t>(off + std::max<int64_t>(bytes, 1)), 0);
   return off;
 }
