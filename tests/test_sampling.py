@@ -238,3 +238,4 @@ def test_vectorised_draws_equal_generator_calls():
                 want = [g.random() for g in real]
                 got = draws.doubles(shots)
             assert list(got) == want
+
