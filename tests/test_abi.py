@@ -147,3 +147,35 @@ def test_no_kernel_spills_to_scratch():
             elif line.startswith("ScratchSize") and int(line.split(":")[-1]) != 0:
                 spills.append((os.path.basename(path), name))
     assert not spills, spills
+
+
+def test_poly_entry_argument_errors_without_device():
+    """pqp_perm_poly_coefficients_c128 validates before it needs a device:
+    order beyond 24, negative exponents, null pointers; rows whose total
+    degree exceeds n are answered (0) without any device work."""
+    import ctypes
+
+    from piquasso_amd import _capi
+
+    L = _capi.lib()
+    dp, i32p, i64p = (ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32),
+                      ctypes.POINTER(ctypes.c_int64))
+    b = np.zeros((2, 2), dtype=np.complex128)
+    bm = np.zeros((1, 2, 2), dtype=np.complex128)
+    out = np.full(4, 7.0)
+    info = np.zeros(6, dtype=np.int64)
+
+    def call(n, exps, b0=b, bms=bm, o=out):
+        e = np.ascontiguousarray(exps, dtype=np.int32)
+        return L.pqp_perm_poly_coefficients_c128(
+            n, e.shape[1], b0.view(np.float64).ctypes.data_as(dp) if b0 is not None else None,
+            bms.view(np.float64).ctypes.data_as(dp), e.shape[0], e.ctypes.data_as(i32p),
+            o.ctypes.data_as(dp) if o is not None else None, info.ctypes.data_as(i64p))
+
+    assert call(25, [[1]]) == _capi.PQP_ERR_RANGE
+    assert call(2, [[-1]]) == _capi.PQP_ERR_INVALID
+    assert call(2, [[1]], b0=None) == _capi.PQP_ERR_INVALID
+    assert call(2, [[1]], o=None) == _capi.PQP_ERR_INVALID
+    assert call(2, [[3], [4]]) == _capi.PQP_OK  # degree > n: zero, no device needed
+    assert list(out) == [0.0, 0.0, 0.0, 0.0]
+    assert info[5] == 3 and info[0] == 0

@@ -36,3 +36,4 @@ def test_connector_class_binds_the_engine(monkeypatch):
     C._cache.clear()
     with pytest.raises(ImportError):
         C.connector_class()
+
