@@ -11,7 +11,10 @@
  * (src/permanent.cpp:49-264, src/n_aryGrayCodeCounter.hpp), on AMD gfx950
  * GPUs.  Every sum and product runs on the GPU; the host only validates,
  * compacts and tabulates.  There is no CPU fallback: without a usable HIP
- * device every compute entry point returns PQP_ERR_DEVICE.
+ * device every compute entry point returns PQP_ERR_DEVICE.  Setting the
+ * environment variable PQP_FORCE_NO_DEVICE (to anything but "" or "0")
+ * before the first call makes the library behave so on a machine that has
+ * a device, without initialising the HIP runtime.
  *
  * Conventions
  *  - Matrices are row-major, interleaved complex (re, im), leading dimension
@@ -63,6 +66,13 @@ const char *pqp_last_error(void);
 /* Device 0 properties: returns PQP_OK and fills the outputs, or
  * PQP_ERR_DEVICE when no HIP device is usable.  `name` may be null. */
 int pqp_device_info(int *n_devices, int *cu_count, char *name, int name_len);
+
+/* Test hook, no counterpart in the reference: with after >= 0 the library
+ * lets `after` more device or pinned-host allocations through and fails the
+ * next one with PQP_ERR_NOMEM (then the hook is off again); after < 0 turns
+ * it off.  Process-wide; always returns PQP_OK.  Buffers are cached and
+ * grow on demand, so a call allocates only what it does not find. */
+int pqp_debug_fail_allocation(int64_t after);
 
 /*
  * One permanent, complex128 in and out.

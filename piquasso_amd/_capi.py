@@ -48,6 +48,7 @@ SIGNATURES = {
     "pqp_version": (_c.c_char_p, []),
     "pqp_last_error": (_c.c_char_p, []),
     "pqp_device_info": (_c.c_int, [_c.POINTER(_c.c_int), _c.POINTER(_c.c_int), _c.c_char_p, _c.c_int]),
+    "pqp_debug_fail_allocation": (_c.c_int, [_c.c_int64]),
     "pqp_permanent_c128": (
         _c.c_int,
         [_dp, _c.c_int64, _c.c_int64, _c.c_int64, _i32p, _i32p, _c.c_uint32, _dp, _dp],

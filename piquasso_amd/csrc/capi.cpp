@@ -9,7 +9,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -31,6 +33,19 @@ thread_local std::string g_error;
 int fail(int code, const std::string &msg) {
   g_error = msg;
   return code;
+}
+
+// Allocation-failure test hook (pqp_debug_fail_allocation): -1 = off; k >= 0
+// lets k more device or pinned-host allocations through, fails the next one
+// with PQP_ERR_NOMEM, and switches itself off.
+std::atomic<int64_t> g_alloc_countdown{-1};
+
+bool forced_alloc_failure() {
+  int64_t c = g_alloc_countdown.load();
+  while (c >= 0) {
+    if (g_alloc_countdown.compare_exchange_weak(c, c - 1)) return c == 0;
+  }
+  return false;
 }
 
 #define PQP_HIP(expr)                                                               \
@@ -58,6 +73,12 @@ DeviceState &device_state() {
   static std::once_flag once;
   std::call_once(once, [] {
     std::memset(st.occ, 0xff, sizeof(st.occ));
+    const char *force = std::getenv("PQP_FORCE_NO_DEVICE");
+    if (force && *force && std::strcmp(force, "0") != 0) {
+      // decided before any HIP call, so the runtime is never initialised
+      st.error = "no HIP device available (PQP_FORCE_NO_DEVICE is set)";
+      return;
+    }
     hipError_t e = hipGetDeviceCount(&st.n_devices);
     if (e != hipSuccess || st.n_devices < 1) {
       st.error = std::string("no HIP device available (") +
@@ -123,6 +144,7 @@ struct DevBuf {
       bytes = 0;
     }
     size_t alloc = std::max<size_t>(n, 4096);
+    if (forced_alloc_failure()) return fail(PQP_ERR_NOMEM, "hipMalloc: forced failure (test hook)");
     hipError_t e = hipMalloc(&p, alloc);
     if (e != hipSuccess) return fail(PQP_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     bytes = alloc;
@@ -148,6 +170,7 @@ struct HostBuf {
       bytes = 0;
     }
     size_t alloc = std::max<size_t>(n, 65536);
+    if (forced_alloc_failure()) return fail(PQP_ERR_NOMEM, "hipHostMalloc: forced failure (test hook)");
     hipError_t e = hipHostMalloc(&p, alloc, hipHostMallocDefault);
     if (e != hipSuccess)
       return fail(PQP_ERR_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
@@ -172,6 +195,7 @@ struct MappedBuf {
     if (n <= bytes) return PQP_OK;
     release();
     size_t alloc = std::max<size_t>(n, 4096);
+    if (forced_alloc_failure()) return fail(PQP_ERR_NOMEM, "hipHostMalloc: forced failure (test hook)");
     hipError_t e = hipHostMalloc(&host, alloc, hipHostMallocMapped | hipHostMallocCoherent);
     if (e != hipSuccess) {
       host = nullptr;
@@ -680,6 +704,11 @@ int pqp_device_info(int *n_devices, int *cu_count, char *name, int name_len) {
   return PQP_OK;
 }
 
+int pqp_debug_fail_allocation(int64_t after) {
+  g_alloc_countdown.store(after < 0 ? -1 : after);
+  return PQP_OK;
+}
+
 int pqp_permanent_c128(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
                        const int32_t *rows, const int32_t *cols, uint32_t flags, double *out,
                        double *out_dd) {
@@ -1091,7 +1120,9 @@ int pqp_batch_create(int64_t count, const void *const *matrices, int is_c64, con
   std::vector<PermInput> inputs(static_cast<size_t>(count));
   for (int64_t k = 0; k < count; ++k)
     inputs[k] = PermInput{matrices[k], is_c64 != 0, m_r[k], m_c[k], ld[k], rows[k], cols[k]};
-  std::unique_ptr<pqp_batch> b(new (std::nothrow) pqp_batch());
+  // every early return (including a failed HIP call) releases what the
+  // batch holds so far
+  std::unique_ptr<pqp_batch, void (*)(pqp_batch *)> b(new (std::nothrow) pqp_batch(), pqp_batch_destroy);
   if (!b) return fail(PQP_ERR_NOMEM, "out of host memory");
   PlanOptions opt;
   opt.fp32_products = (flags & PQP_FLAG_FP32) != 0 || is_c64 != 0;
@@ -1128,19 +1159,13 @@ int pqp_batch_create(int64_t count, const void *const *matrices, int is_c64, con
   b->x.flags = flags;
   PQP_HIP(hipStreamCreateWithFlags(&b->x.stream, hipStreamNonBlocking));
   b->x.own_stream = true;
-  if ((st = upload(b->x)) != PQP_OK) {
-    pqp_batch_destroy(b.release());
-    return st;
-  }
+  if ((st = upload(b->x)) != PQP_OK) return st;
   if (b->grad) {
     index_units(b->xm);
     b->xm.flags = PQP_FLAG_STATIC;
     b->xm.stream = b->x.stream;  // shares the batch stream (and its graph)
     b->xm.own_stream = false;
-    if ((st = upload(b->xm)) != PQP_OK) {
-      pqp_batch_destroy(b.release());
-      return st;
-    }
+    if ((st = upload(b->xm)) != PQP_OK) return st;
   }
   PQP_HIP(hipStreamSynchronize(b->x.stream));
   *out = b.release();
@@ -1288,6 +1313,7 @@ void pqp_batch_destroy(pqp_batch *b) {
   b->xm.arena.release();
   b->xm.partials.release();
   b->xm.results.release();
+  b->xm.results_m.release();
   b->xm.stage.release();
   b->xm.heads.release();
   for (int i = 1; i < kMaxRanges; ++i) {
@@ -1298,6 +1324,7 @@ void pqp_batch_destroy(pqp_batch *b) {
   b->x.arena.release();
   b->x.partials.release();
   b->x.results.release();
+  b->x.results_m.release();
   b->x.stage.release();
   b->x.heads.release();
   for (int i = 1; i < kMaxRanges; ++i) {

@@ -179,3 +179,42 @@ def test_poly_entry_argument_errors_without_device():
     assert call(2, [[3], [4]]) == _capi.PQP_OK  # degree > n: zero, no device needed
     assert list(out) == [0.0, 0.0, 0.0, 0.0]
     assert info[5] == 3 and info[0] == 0
+
+
+_NO_DEVICE_CHILD = r"""
+import ctypes, numpy as np
+from piquasso_amd import _capi
+import piquasso_amd as pa
+L = _capi.lib()
+n = ctypes.c_int(-1)
+print("status", L.pqp_device_info(ctypes.byref(n), None, None, 0), n.value)
+print("message", L.pqp_last_error().decode())
+try:
+    pa.permanent(np.eye(2, dtype=complex), np.array([1, 1]), np.array([1, 1]))
+    print("computed")
+except RuntimeError as e:
+    print("raised", e)
+"""
+
+
+def test_forced_no_device_switch():
+    """PQP_FORCE_NO_DEVICE makes every call fail with PQP_ERR_DEVICE, named
+    as forced, before the HIP runtime is touched (a fresh process: the
+    switch is read once)."""
+    import subprocess
+    import sys
+
+    env = dict(os.environ, PQP_FORCE_NO_DEVICE="1")
+    res = subprocess.run([sys.executable, "-c", _NO_DEVICE_CHILD], cwd=ROOT, env=env,
+                         capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stderr
+    lines = res.stdout.splitlines()
+    assert lines[0] == f"status {_capi.PQP_ERR_DEVICE} 0"
+    assert "PQP_FORCE_NO_DEVICE" in lines[1]
+    assert lines[2].startswith("raised") and "PQP_FORCE_NO_DEVICE" in lines[2]
+
+
+def test_allocation_hook_arms_and_disarms():
+    L = _capi.lib()
+    assert L.pqp_debug_fail_allocation(3) == 0
+    assert L.pqp_debug_fail_allocation(-1) == 0
