@@ -7,6 +7,7 @@
 #include "pqp_permanent.h"
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <atomic>
@@ -47,6 +48,15 @@ bool forced_alloc_failure() {
   }
   return false;
 }
+
+// One roctx range per compute call, named after the entry point: under
+// rocprofv3 --marker-trace the kernels and copies of a call fall inside it.
+struct CallRange {
+  explicit CallRange(const char *name) { roctxRangePushA(name); }
+  ~CallRange() { roctxRangePop(); }
+  CallRange(const CallRange &) = delete;
+  CallRange &operator=(const CallRange &) = delete;
+};
 
 #define PQP_HIP(expr)                                                               \
   do {                                                                              \
@@ -712,6 +722,7 @@ int pqp_debug_fail_allocation(int64_t after) {
 int pqp_permanent_c128(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
                        const int32_t *rows, const int32_t *cols, uint32_t flags, double *out,
                        double *out_dd) {
+  CallRange range_("pqp_permanent_c128");
   if (!out) return fail(PQP_ERR_INVALID, "out is null");
   PermInput in{matrix, false, m_r, m_c, ld, rows, cols};
   return run_single(in, flags, out, out_dd);
@@ -720,6 +731,7 @@ int pqp_permanent_c128(const double *matrix, int64_t m_r, int64_t m_c, int64_t l
 int pqp_permanent_c64(const float *matrix, int64_t m_r, int64_t m_c, int64_t ld,
                       const int32_t *rows, const int32_t *cols, uint32_t flags, float *out,
                       double *out_dd) {
+  CallRange range_("pqp_permanent_c64");
   if (!out) return fail(PQP_ERR_INVALID, "out is null");
   PermInput in{matrix, true, m_r, m_c, ld, rows, cols};
   double o[2];
@@ -733,6 +745,7 @@ int pqp_permanent_c64(const float *matrix, int64_t m_r, int64_t m_c, int64_t ld,
 
 int pqp_laplace_c128(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
                      const int32_t *rows, const int32_t *cols, double *out) {
+  CallRange range_("pqp_laplace_c128");
   if (!out && m_c > 0) return fail(PQP_ERR_INVALID, "out is null");
   PermInput in{matrix, false, m_r, m_c, ld, rows, cols};
   PlanOptions opt;
@@ -776,6 +789,7 @@ int pqp_laplace_c128(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
 int pqp_laplace_batch_c128(int64_t count, const double *const *matrices, const int64_t *m_r,
                            const int64_t *m_c, const int64_t *ld, const int32_t *const *rows,
                            const int32_t *const *cols, double *const *outs) {
+  CallRange range_("pqp_laplace_batch_c128");
   if (count < 0 || (count > 0 && (!matrices || !m_r || !m_c || !ld || !rows || !cols || !outs)))
     return fail(PQP_ERR_INVALID, "invalid laplace batch arguments");
   if (count > INT32_MAX) return fail(PQP_ERR_RANGE, "too many items");
@@ -833,6 +847,7 @@ int pqp_laplace_batch_c128(int64_t count, const double *const *matrices, const i
 int pqp_tensor_permanent_batch_c128(int64_t count, const double *const *matrices, const int64_t *n,
                                     const int64_t *ld, const double *overlap, double *out,
                                     double *abs_sum) {
+  CallRange range_("pqp_tensor_permanent_batch_c128");
   if (count < 0 || (count > 0 && (!matrices || !n || !ld || !overlap || !out)))
     return fail(PQP_ERR_INVALID, "invalid tensor permanent arguments");
   if (count > INT32_MAX) return fail(PQP_ERR_RANGE, "too many items");
@@ -907,6 +922,7 @@ int pqp_perm_poly_plan_info(int64_t n, int64_t n_vars, int64_t n_out, const int3
 int pqp_perm_poly_coefficients_c128(int64_t n, int64_t n_vars, const double *b0, const double *bm,
                                     int64_t n_out, const int32_t *exponents, double *out,
                                     int64_t *info) {
+  CallRange range_("pqp_perm_poly_coefficients_c128");
   if (n_out < 0 || (n_out > 0 && !out) || (n >= 1 && !b0) || (n_vars > 0 && n >= 1 && !bm))
     return fail(PQP_ERR_INVALID, "invalid matrix polynomial arguments");
   PolyState &S = poly_state();
@@ -1053,6 +1069,7 @@ int pqp_perm_poly_coefficients_c128(int64_t n, int64_t n_vars, const double *b0,
 int pqp_grad_batch_c128(int64_t count, const double *const *matrices, const int64_t *m_r,
                         const int64_t *m_c, const int64_t *ld, const int32_t *const *rows,
                         const int32_t *const *cols, double *perms, double *const *grads) {
+  CallRange range_("pqp_grad_batch_c128");
   if (count < 0 || (count > 0 && (!matrices || !m_r || !m_c || !ld || !rows || !cols || !grads)))
     return fail(PQP_ERR_INVALID, "invalid gradient batch arguments");
   std::vector<PermInput> inputs;
@@ -1112,6 +1129,7 @@ int pqp_batch_create(int64_t count, const void *const *matrices, int is_c64, con
                      const int64_t *m_c, const int64_t *ld, const int32_t *const *rows,
                      const int32_t *const *cols, uint32_t flags, int64_t target_inner,
                      int64_t total_units, pqp_batch **out) {
+  CallRange range_("pqp_batch_create");
   if (!out) return fail(PQP_ERR_INVALID, "out is null");
   *out = nullptr;
   if (count < 0 || (count > 0 && (!matrices || !m_r || !m_c || !ld || !rows || !cols)))
@@ -1173,6 +1191,7 @@ int pqp_batch_create(int64_t count, const void *const *matrices, int is_c64, con
 }
 
 int pqp_batch_run(pqp_batch *b, int mode) {
+  CallRange range_("pqp_batch_run");
   if (!b) return fail(PQP_ERR_INVALID, "batch is null");
   int st;
   if (mode == PQP_RUN_GRAPH) {
@@ -1200,12 +1219,14 @@ int pqp_batch_run(pqp_batch *b, int mode) {
 }
 
 int pqp_batch_sync(pqp_batch *b) {
+  CallRange range_("pqp_batch_sync");
   if (!b) return fail(PQP_ERR_INVALID, "batch is null");
   PQP_HIP(hipStreamSynchronize(b->x.stream));
   return PQP_OK;
 }
 
 int pqp_batch_results(pqp_batch *b, double *out, double *out_dd) {
+  CallRange range_("pqp_batch_results");
   if (!b) return fail(PQP_ERR_INVALID, "batch is null");
   if (!b->ran) return fail(PQP_ERR_INVALID, "batch has not been run");
   const size_t n = b->x.plan.items.size();
@@ -1218,6 +1239,7 @@ int pqp_batch_results(pqp_batch *b, double *out, double *out_dd) {
 }
 
 int pqp_batch_time(pqp_batch *b, int mode, int reps, float *ms) {
+  CallRange range_("pqp_batch_time");
   if (!b || reps < 0 || (reps > 0 && !ms)) return fail(PQP_ERR_INVALID, "invalid timing arguments");
   hipEvent_t e0, e1;
   PQP_HIP(hipEventCreate(&e0));
@@ -1268,6 +1290,7 @@ int pqp_batch_stats(const pqp_batch *b, double *flops, int64_t *terms, int64_t *
 }
 
 int pqp_batch_gradients(pqp_batch *b, double *const *grads) {
+  CallRange range_("pqp_batch_gradients");
   if (!b || !grads) return fail(PQP_ERR_INVALID, "invalid arguments");
   if (!b->grad) return fail(PQP_ERR_INVALID, "batch was not created with PQP_FLAG_GRADIENT");
   if (!b->ran) return fail(PQP_ERR_INVALID, "batch has not been run");
