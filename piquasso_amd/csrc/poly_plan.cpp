@@ -81,9 +81,15 @@ struct MonoSet {
 // when M * bits <= 64): hashing and the step a -> a - e_v are single
 // integer operations.
 struct PackedSet {
+  // open addressing with the key stored beside its insertion index, so a
+  // probe touches one cache line instead of the slot and then keys[]
+  struct Slot {
+    uint64_t key;
+    int64_t idx;  // -1 = empty
+  };
   int bits;
-  std::vector<uint64_t> keys;   // by insertion index
-  std::vector<int32_t> table;   // -1 = empty, else an insertion index
+  std::vector<uint64_t> keys;  // by insertion index
+  std::vector<Slot> table;
   size_t mask = 0;
   int shift = 0;
 
@@ -97,29 +103,29 @@ struct PackedSet {
   size_t slot(uint64_t k) const { return static_cast<size_t>((k * 0x9E3779B97F4A7C15ull) >> shift); }
 
   void rehash(size_t cap) {
-    table.assign(cap, -1);
+    table.assign(cap, Slot{0, -1});
     mask = cap - 1;
     shift = 64 - __builtin_ctzll(cap);
     for (size_t k = 0; k < keys.size(); ++k) {
       size_t s = slot(keys[k]);
-      while (table[s] >= 0) s = (s + 1) & mask;
-      table[s] = static_cast<int32_t>(k);
+      while (table[s].idx >= 0) s = (s + 1) & mask;
+      table[s] = Slot{keys[k], static_cast<int64_t>(k)};
     }
   }
 
   int64_t find_or_add(uint64_t key, bool *added) {
     size_t s = slot(key);
-    while (table[s] >= 0) {
-      if (keys[table[s]] == key) {
+    while (table[s].idx >= 0) {
+      if (table[s].key == key) {
         *added = false;
-        return table[s];
+        return table[s].idx;
       }
       s = (s + 1) & mask;
     }
     *added = true;
     const int64_t k = static_cast<int64_t>(keys.size());
     keys.push_back(key);
-    table[s] = static_cast<int32_t>(k);
+    table[s] = Slot{key, k};
     if (keys.size() * 2 > table.size()) rehash(table.size() * 2);
     return k;
   }
