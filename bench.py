@@ -261,10 +261,15 @@ def _cpu_sampling(seconds: float):
 
 def run_cpu_baseline(seconds: float):
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = max(1, min(16, cores or 1))
+    cores = max(1, cores or 1)
+    # every host core this process may run on, unless the host's CPU share
+    # for this GPU is set through OMP_NUM_THREADS (16 threads per GPU on the
+    # MI355X hosts, whatever the affinity mask shows)
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(cores, int(share)) if share.isdigit() and int(share) > 0 else cores
     env = dict(os.environ)
     # the reference asks for 4 x hardware_concurrency OpenMP threads per call
-    # (src/permanent.cpp:144-153); cap them at the host's CPU share
+    # (src/permanent.cpp:144-153); cap them at the threads above
     env["OMP_THREAD_LIMIT"] = str(threads)
     env["OMP_NUM_THREADS"] = str(threads)
     try:
@@ -274,13 +279,15 @@ def run_cpu_baseline(seconds: float):
         )
         rec = json.loads(out.stdout.strip().splitlines()[-1])
     except Exception as e:  # the GPU numbers are still worth reporting
-        return {"value": None, "unit": "permanents/s", "cores": threads, "kind": "reference",
-                "sample": f"failed: {e}"}
+        return {"value": None, "unit": "permanents/s", "cores": threads, "nproc": cores,
+                "kind": "reference", "sample": f"failed: {e}"}
     rate = rec["items"] / rec["seconds"] if rec["seconds"] > 0 else None
     return {
         "value": rate,
         "unit": "permanents/s",
         "cores": threads,
+        "nproc": cores,
+        "omp_num_threads": share or None,
         "kind": rec["kind"],
         "sample": (f"{rec['items']} permanents of the batched workload, in order, "
                    f"cycling over its 4096 items ({rec['flops']:.3e} algorithmic FLOP) in "

@@ -103,10 +103,15 @@ int pqp_permanent_c64(const float *matrix, int64_t m_r, int64_t m_c, int64_t ld,
                       double *out_dd);
 
 /*
- * Batches: many permanents of uneven size in one persistent, work-stealing
- * launch (the reference has no batched entry; each item's value equals
- * pqp_permanent_c128 on that item).  The batch owns its device buffers and
- * stream; inputs are copied in at creation.
+ * Batches: many permanents of uneven size, one launch per column-width
+ * bucket (the buckets run concurrently on forked streams), then one finalize
+ * launch; the reference has no batched entry, and each item's value equals
+ * pqp_permanent_c128 on that item.  Each bucket's launch is a persistent,
+ * work-stealing kernel unless flags has PQP_FLAG_STATIC, which launches one
+ * block per work unit in decreasing cost order instead (faster on MI355X
+ * for the measured batches; the default of the Python layer's autotuned
+ * configuration).  The batch owns its device buffers and stream; inputs are
+ * copied in at creation.
  *   matrices : count pointers, complex128 (is_c64 = 0) or complex64 (1)
  *   m_r, m_c, ld : count entries each
  *   rows, cols   : count pointers to int32 multiplicities
@@ -122,7 +127,7 @@ int pqp_batch_create(int64_t count, const void *const *matrices, int is_c64, con
                      int64_t total_units, pqp_batch **out);
 
 /* Run modes for pqp_batch_run / pqp_batch_time */
-#define PQP_RUN_EAGER 0  /* persistent kernel(s) + finalize, launched on the batch stream */
+#define PQP_RUN_EAGER 0  /* the sweep kernel(s) + finalize, launched directly on the batch stream */
 #define PQP_RUN_GRAPH 1  /* the same launches captured once in a hipGraph and replayed */
 #define PQP_RUN_PLAIN 2  /* plain FP64 kernel, one launch (+ finalize) per permanent */
 #define PQP_RUN_KERNEL 3 /* timing only: the sweep kernel(s) alone, finalize outside the events */

@@ -923,7 +923,8 @@ int pqp_perm_poly_coefficients_c128(int64_t n, int64_t n_vars, const double *b0,
                                     int64_t n_out, const int32_t *exponents, double *out,
                                     int64_t *info) {
   CallRange range_("pqp_perm_poly_coefficients_c128");
-  if (n_out < 0 || (n_out > 0 && !out) || (n >= 1 && !b0) || (n_vars > 0 && n >= 1 && !bm))
+  if (n_out < 0 || (n_out > 0 && !out) || (n >= 1 && !b0) || (n_vars > 0 && n >= 1 && !bm) ||
+      (n_out > 0 && n_vars > 0 && !exponents))
     return fail(PQP_ERR_INVALID, "invalid matrix polynomial arguments");
   PolyState &S = poly_state();
   std::lock_guard<std::mutex> lock(S.mu);

@@ -7,7 +7,15 @@ double-double oracle (oracle/glynn_dd.c), which tests/test_oracle.py pins
 against exact rational brute force, Ryser's formula and the reference's own
 vectors.  The n = 32 case takes a few minutes on 8 cores.
 
+With --reference the script instead adds, to the cases already in the file,
+the output of the reference's own pybind ``permanent()`` built from its
+sources into oracle/_ref (``make -C oracle ref``), and the number of host
+threads it ran with: its chunking, hence its last bits, depends on that
+count (src/permanent.cpp:144-165).  The GPU tests compare against both
+values.
+
 Run:  python tests/golden/make_extended_vectors.py [--skip-32]
+      python tests/golden/make_extended_vectors.py --reference
 """
 
 import json
@@ -46,7 +54,39 @@ def entry(name, a, rows, cols, note):
     }
 
 
+def add_reference_outputs(path):
+    """Record the reference build's permanent() beside each stored dd value."""
+    mod = O.ref_module()
+    with open(path) as f:
+        data = json.load(f)
+    threads = os.cpu_count()
+    for case in data["cases"]:
+        m = case["matrix"]
+        a = (np.asarray(m["re"]) + 1j * np.asarray(m["im"])).reshape(m["shape"])
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        rows = np.asarray(case["rows"], dtype=np.int64)
+        cols = np.asarray(case["cols"], dtype=np.int64)
+        t0 = time.time()
+        val = complex(mod.permanent(a, rows=rows, cols=cols))
+        dt = time.time() - t0
+        dd = case["dd"]
+        exact = complex(dd[0] + dd[1], dd[2] + dd[3])
+        err = abs(val - exact) / abs(exact)
+        print(f"{case['id']}: reference {val:.17e}  |ref - dd| / |dd| = {err:.2e}  ({dt:.1f} s)",
+              flush=True)
+        case["reference"] = {"re": val.real, "im": val.imag, "host_threads": threads,
+                             "seconds": dt, "rel_err_vs_dd": err}
+    data["reference"] = ("the reference's pybind permanent() compiled from its sources "
+                         "(oracle/Makefile ref), run in the build container")
+    with open(path, "w") as f:
+        json.dump(data, f)
+    print("wrote", path)
+
+
 def main():
+    if "--reference" in sys.argv:
+        add_reference_outputs(os.path.join(HERE, "extended_vectors.json"))
+        return
     skip32 = "--skip-32" in sys.argv
     cases = []
     a, r, c = W.plain_single(seed=1)

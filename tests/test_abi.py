@@ -58,7 +58,20 @@ def test_version_and_device_query_without_gpu():
     (np.int64, np.complex128), (np.int32, np.complex128),
 ])
 def test_dtype_dispatch_mirrors_pybind(dtype, expect):
-    assert resolve_dtype(np.zeros((2, 2), dtype=dtype)) == expect
+    """The complex dtype the drop-in computes in, for each input dtype, is the
+    one the reference's pybind overloads pick (piquasso/_math/permanent.cpp:73-78):
+    compared with the reference module built from its sources when that
+    build exists (oracle/_ref), else with the expected table."""
+    a = np.eye(2, dtype=dtype)
+    assert resolve_dtype(a) == expect
+    from oracle import oracle as O
+
+    try:
+        ref = O.ref_module().permanent
+    except O.OracleUnavailable:
+        return
+    out = ref(a, rows=np.ones(2, dtype=np.int64), cols=np.ones(2, dtype=np.int64))
+    assert out.dtype == expect
 
 
 def test_dtype_dispatch_lists_and_errors():
@@ -179,6 +192,12 @@ def test_poly_entry_argument_errors_without_device():
     assert call(2, [[3], [4]]) == _capi.PQP_OK  # degree > n: zero, no device needed
     assert list(out) == [0.0, 0.0, 0.0, 0.0]
     assert info[5] == 3 and info[0] == 0
+    # the same sizes again (the plan of the previous call is cached) with a
+    # null exponent pointer: rejected before the cache is consulted
+    st = L.pqp_perm_poly_coefficients_c128(2, 1, b.view(np.float64).ctypes.data_as(dp),
+                                           bm.view(np.float64).ctypes.data_as(dp), 2, None,
+                                           out.ctypes.data_as(dp), info.ctypes.data_as(i64p))
+    assert st == _capi.PQP_ERR_INVALID
 
 
 _NO_DEVICE_CHILD = r"""

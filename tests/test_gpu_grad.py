@@ -217,6 +217,42 @@ def test_torch_autograd_matches_finite_differences(gpu):
     assert np.allclose(got, 2 * pv * np.conj(G), rtol=1e-13, atol=1e-15)
 
 
+@pytest.mark.parametrize("n_want", [12, 16, 20])
+def test_gradient_workload_items_against_finite_differences(gpu, n_want):
+    """Items of the 256-gradient workload (workloads.gradient_batch(seed=5),
+    n ~ U{12..20}, Haar submatrices) at n = 12, 16 and 20: the batched torch
+    op's A.grad for L = |perm(A)|^2 against central differences of L in the
+    real and imaginary part of every entry (h = 1e-6, forward permanents on
+    the GPU), and against 2 perm conj(G) from the batch."""
+    torch = pytest.importorskip("torch")
+    mats, rows, cols = W.gradient_batch(count=256, seed=5)
+    k = next(i for i, r in enumerate(rows) if int(np.sum(r)) == n_want)
+    a, r, c = mats[k], rows[k], cols[k]
+    print(f"gradient workload item {k}: n = {n_want}", flush=True)
+    A = torch.tensor(a[None], dtype=torch.complex128, requires_grad=True)
+    p = pa.torch_permanent_batch(A, r[None], c[None])
+    loss = (p * p.conj()).real.sum()
+    loss.backward()
+    got = A.grad[0].numpy()
+
+    def L(m):
+        return abs(complex(pa.permanent(m, r, c))) ** 2
+
+    h = 1e-6
+    m_r, m_c = a.shape
+    fd = np.zeros_like(a)
+    for i in range(m_r):
+        for j in range(m_c):
+            e = np.zeros_like(a)
+            e[i, j] = h
+            fd[i, j] = (L(a + e) - L(a - e)) / (2 * h) + 1j * (L(a + 1j * e) - L(a - 1j * e)) / (2 * h)
+    scale = np.abs(fd).max()
+    assert np.abs(got - fd).max() <= 1e-6 * scale, (k, np.abs(got - fd).max() / scale)
+    _, grads = pa.permanent_grad_batch([a], [r], [c])
+    pv = complex(pa.permanent(a, r, c))
+    assert np.allclose(got, 2 * pv * np.conj(grads[0]), rtol=1e-12, atol=1e-14 * scale)
+
+
 def test_torch_batched_autograd(gpu):
     """L = sum_b w_b |perm(A_b)|^2 over a stack: the batched op's gradient
     equals the single-matrix op's, item by item, and its values the
