@@ -90,6 +90,24 @@ int pqp_permanent_c128(const double *matrix, int64_t m_r, int64_t m_c, int64_t l
                        double *out, double *out_dd);
 
 /*
+ * pqp_permanent_c128 without allocating: no hipMalloc, hipFree or pinned
+ * host allocation in the call.  No counterpart in the reference: this entry
+ * is for callers that cannot allocate device memory (serving loops and
+ * harnesses that forbid it).  Same arguments, flags and result as
+ * pqp_permanent_c128, from the same kernels and plan (bit-identical).  The
+ * device memory the call needs (plan tables, per-wave partial sums, result
+ * slots) comes from a fixed 8 MiB device array that each GPU holds for the
+ * life of the process; that holds permanents of up to 2^39 Gray terms
+ * (n = 40 at unit multiplicities), and one needing more returns
+ * PQP_ERR_RANGE with the size in the message.  Calls serialise on one lock; the array keeps
+ * the last call's contents, and nothing in it is assumed to start at zero.
+ * Runs on the null stream with synchronous copies from and to host memory.
+ */
+int pqp_permanent_c128_noalloc(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
+                               const int32_t *rows, const int32_t *cols, uint32_t flags,
+                               double *out, double *out_dd);
+
+/*
  * One permanent, complex64 in and out.
  * Replaces permanent_cpp<float> (src/permanent.cpp:266-267), the overload
  * the pybind entry picks for complex64 input (piquasso/_math/permanent.cpp:73-75).

@@ -53,6 +53,10 @@ SIGNATURES = {
         _c.c_int,
         [_dp, _c.c_int64, _c.c_int64, _c.c_int64, _i32p, _i32p, _c.c_uint32, _dp, _dp],
     ),
+    "pqp_permanent_c128_noalloc": (
+        _c.c_int,
+        [_dp, _c.c_int64, _c.c_int64, _c.c_int64, _i32p, _i32p, _c.c_uint32, _dp, _dp],
+    ),
     "pqp_permanent_c64": (
         _c.c_int,
         [_fp, _c.c_int64, _c.c_int64, _c.c_int64, _i32p, _i32p, _c.c_uint32, _fp, _dp],

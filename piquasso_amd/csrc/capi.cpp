@@ -20,6 +20,7 @@
 #include <string>
 #include <vector>
 
+#include "capi_internal.hpp"
 #include "perm_kernels.hpp"
 #include "plan.hpp"
 #include "poly_plan.hpp"
@@ -437,14 +438,7 @@ SingleState &single_state() {
 int run_single(const PermInput &in, uint32_t flags, double *out2, double *out_dd4) {
   // Validate and plan first: argument errors are reported the same way with
   // or without a device.
-  PlanOptions opt;
-  opt.fp32_products = (flags & PQP_FLAG_FP32) != 0;
-  opt.expand_cols = expand_mode(flags);
-  const DeviceState &dev = device_state();
-  // waves per permanent: ~1.4 sqrt(T), at most kSingleUnitsPerCu per CU;
-  // a permanent of one wave writes its result from the sweep kernel
-  opt.auto_units_cap = static_cast<int64_t>(std::max(dev.cus, 1)) * kSingleUnitsPerCu;
-  opt.direct_out = (flags & PQP_FLAG_PLAIN) == 0;
+  const PlanOptions opt = single_plan_options(flags);
   Plan plan;
   std::string err;
   int st = plan_build(std::vector<PermInput>{in}, opt, plan, err);
@@ -695,6 +689,26 @@ PolyState &poly_state() {
 size_t align256(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
 
 }  // namespace
+
+namespace pqp {
+
+int capi_fail(int code, const std::string &msg) { return fail(code, msg); }
+
+int capi_require_device() { return require_device(); }
+
+PlanOptions single_plan_options(uint32_t flags) {
+  PlanOptions opt;
+  opt.fp32_products = (flags & PQP_FLAG_FP32) != 0;
+  opt.expand_cols = expand_mode(flags);
+  const DeviceState &dev = device_state();
+  // waves per permanent: ~1.4 sqrt(T), at most kSingleUnitsPerCu per CU;
+  // a permanent of one wave writes its result from the sweep kernel
+  opt.auto_units_cap = static_cast<int64_t>(std::max(dev.cus, 1)) * kSingleUnitsPerCu;
+  opt.direct_out = (flags & PQP_FLAG_PLAIN) == 0;
+  return opt;
+}
+
+}  // namespace pqp
 
 extern "C" {
 

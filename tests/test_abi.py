@@ -200,6 +200,31 @@ def test_poly_entry_argument_errors_without_device():
     assert st == _capi.PQP_ERR_INVALID
 
 
+def test_noalloc_entry_errors_without_device():
+    """pqp_permanent_c128_noalloc checks its arguments and its scratch limit
+    before it needs a device: a null output, a sum mismatch, and a permanent
+    of 2^40 Gray terms (n = 41), whose plan exceeds the 8 MiB device array."""
+    L = _capi.lib()
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    dp = ctypes.POINTER(ctypes.c_double)
+
+    def call(a, rows, cols, out=True):
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        c = np.ascontiguousarray(cols, dtype=np.int32)
+        o = np.zeros(2)
+        return L.pqp_permanent_c128_noalloc(a.view(np.float64).ctypes.data_as(dp), a.shape[0],
+                                            a.shape[1], a.shape[1], r.ctypes.data_as(i32p),
+                                            c.ctypes.data_as(i32p), 0,
+                                            o.ctypes.data_as(dp) if out else None, None)
+
+    assert call(np.eye(2), [1, 1], [1, 1], out=False) == _capi.PQP_ERR_INVALID
+    assert call(np.eye(2), [1, 1], [2, 1]) == _capi.PQP_ERR_SUM_MISMATCH
+    assert call(np.zeros((41, 41)), np.ones(41), np.ones(41)) == _capi.PQP_ERR_RANGE
+    msg = L.pqp_last_error().decode()
+    assert "bytes of device scratch" in msg and "8388608" in msg
+
+
 _NO_DEVICE_CHILD = r"""
 import ctypes, numpy as np
 from piquasso_amd import _capi
