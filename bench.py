@@ -322,9 +322,12 @@ def _traffic_from_profile():
 
 def _gradient_leg(pa, W, reps: int, cpu=None):
     """Gradients of 256 permanents, n ~ U{12..20}, unit multiplicities, Haar
-    48-mode submatrices: forward sweep + one column-minor sweep per row, all
-    in one hipGraph.  Euler's identity sum_ij A_ij G_ij = n perm(A) checks
-    every gradient (relative to sum_ij |A_ij G_ij|)."""
+    48-mode submatrices: one Gray sweep per permanent gives the permanent and
+    all n^2 entries of its gradient, in one hipGraph.  Euler's identity
+    sum_ij A_ij G_ij = n perm(A) checks every gradient (relative to
+    sum_ij |A_ij G_ij|).  TFLOP/s by the model F = sum T (8 m_c + 18 (n - 1)
+    + 4) per gradient (row update, exclusion products, accumulation, one row
+    flush and the weight per Gray term)."""
     mats, rows, cols = W.gradient_batch(count=256, seed=5)
     with pa.PermanentBatch(mats, rows, cols, gradient=True) as gb:
         gb.run("graph")
@@ -351,8 +354,10 @@ def _gradient_leg(pa, W, reps: int, cpu=None):
         "ms": med,
         "tflops": st["flops"] / (med * 1e-3) / 1e12,
         "euler_residual_max": float(resid),
-        "what": "256 gradients d perm/dA (n~U{12..20}, 48-mode Haar submatrices): permanent "
-                "sweep + one column-minor sweep per row, one hipGraph replay",
+        "fp64_peak_frac": st["flops"] / (med * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+        "flop_model": "sum over items of T * (8 m_c + 18 (n - 1) + 4), T = Gray terms",
+        "what": "256 gradients d perm/dA (n~U{12..20}, 48-mode Haar submatrices): one Gray sweep "
+                "per permanent for the permanent and all n^2 gradient entries, one hipGraph replay",
     }
 
 

@@ -283,8 +283,11 @@ int pqp_grad_batch_c128(int64_t count, const double *const *matrices, const int6
  * the ItemDesc record at offset 0, then the work units and tables at the
  * offsets it names (piquasso_amd/csrc/pqp_types.hpp).  With buf == null only
  * *needed_bytes is written.  flags & PQP_FLAG_GRADIENT plans the column-minor
- * sweep of pqp_laplace_c128 instead.  No counterpart in the reference.
+ * sweep of pqp_laplace_c128 instead, flags & PQP_PLAN_GRADIENT_SWEEP the
+ * single-sweep gradient item of pqp_grad_batch_c128.  No counterpart in the
+ * reference.
  */
+#define PQP_PLAN_GRADIENT_SWEEP 0x40u
 int pqp_plan_image(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
                    const int32_t *rows, const int32_t *cols, uint32_t flags,
                    int64_t target_inner, int64_t min_units, void *buf, int64_t buf_bytes,

@@ -118,6 +118,10 @@ DeviceState &device_state() {
 // items of n = 8..24: 8192 units; finer splits add per-unit start cost).
 constexpr int64_t kSingleUnitsPerCu = 128;
 constexpr int64_t kBatchUnitsPerCu = 32;
+// Gradient sweeps write every row's gradient per unit and run two waves per
+// SIMD: about 8 units per CU over the batch measured best (256 gradients of
+// n = 12..20: 0.74 ms at 8 or 16 per CU, 0.91 ms at 32).
+constexpr int64_t kGradUnitsPerCu = 8;
 
 int expand_mode(uint32_t flags) {
   if (flags & PQP_FLAG_EXPAND_COLS) return 1;
@@ -319,7 +323,8 @@ int enqueue(Exec &x, int mode, hipEvent_t ev_begin, hipEvent_t ev_end) {
   const int n_items = static_cast<int>(p.items.size());
   const bool kernel_only = (mode == PQP_RUN_KERNEL);
   if (ev_begin) PQP_HIP(hipEventRecord(ev_begin, x.stream));
-  if (p.minors && mode == PQP_RUN_PLAIN) return fail(PQP_ERR_INVALID, "no plain mode for column minors");
+  if ((p.minors || p.grad) && mode == PQP_RUN_PLAIN)
+    return fail(PQP_ERR_INVALID, "no plain mode for column minors or gradients");
   if (mode == PQP_RUN_PLAIN) {
     for (int k = 0; k < n_items; ++k) {
       const ItemDesc &d = p.items[k];
@@ -370,7 +375,7 @@ int enqueue(Exec &x, int mode, hipEvent_t ev_begin, hipEvent_t ev_end) {
       l.fp32 = p.fp32;
       l.comp = !(x.flags & PQP_FLAG_PLAIN) || p.fp32;
       l.stream = s;
-      if (l.persistent && !p.minors) {
+      if (l.persistent && !p.minors && !p.grad) {
         // resident waves in proportion to the range's share of the work, so
         // concurrent ranges run side by side instead of one after another
         const int per_cu = blocks_per_cu(r.bucket, l.fp32, l.comp, true);
@@ -378,7 +383,9 @@ int enqueue(Exec &x, int mode, hipEvent_t ev_begin, hipEvent_t ev_end) {
         const int slots = static_cast<int>(std::ceil(share * dev.cus * per_cu));
         l.grid = std::max(1, std::min(r.count, std::max(slots, dev.cus / 4)));
       }
-      if (p.minors)
+      if (p.grad)
+        PQP_HIP(launch_grad(r.bucket, a, r.first, r.count, s));
+      else if (p.minors)
         PQP_HIP(launch_minors(r.bucket, a, r.first, r.count, s));
       else
         PQP_HIP(launch_fwd(r.bucket, l));
@@ -389,7 +396,12 @@ int enqueue(Exec &x, int mode, hipEvent_t ev_begin, hipEvent_t ev_end) {
     }
     if (kernel_only && ev_end) PQP_HIP(hipEventRecord(ev_end, x.stream));
     const bool comp = !(x.flags & PQP_FLAG_PLAIN) || p.fp32;
-    if (p.all_direct && !p.minors) {
+    if (p.grad) {
+      int max_values = 1;
+      for (const ItemDesc &d : p.items)
+        if (d.n > 0) max_values = std::max(max_values, 1 + (1 + d.L + d.Dt) * d.mc);
+      PQP_HIP(launch_grad_finalize(a, n_items, max_values, x.stream));
+    } else if (p.all_direct && !p.minors) {
       // every item was a single unit that wrote its own result
     } else if (p.minors)
       PQP_HIP(launch_minors_finalize(a, n_items, p.max_units, x.stream));
@@ -405,9 +417,11 @@ int enqueue(Exec &x, int mode, hipEvent_t ev_begin, hipEvent_t ev_end) {
 
 struct pqp_batch {
   Exec x;
-  // gradient batches: the column-minor items (one per row of non-zero
-  // multiplicity of every permanent) run after the permanents on x's stream
+  // gradient batches: x runs the gradient plan (the permanent and its
+  // gradient of every item from one sweep); with grad_minors (permanents
+  // above 32 particles) x runs the permanents and xm the column minors
   bool grad = false;
+  bool grad_minors = false;
   Exec xm;
   std::vector<std::vector<int32_t>> minor_rows;  // r - e_i of every minor item
   std::vector<int64_t> minor_item, minor_row;    // (permanent, row) of each minor item
@@ -471,7 +485,8 @@ int run_single(const PermInput &in, uint32_t flags, double *out2, double *out_dd
   return PQP_OK;
 }
 
-// Column-minor inputs of the gradients of `inputs`: one item per row with
+// Gradients beyond the single-sweep kernel (more than 32 particles): the
+// column-minor inputs, one item per row with
 // non-zero multiplicity, rows r - e_i, columns c.
 void minor_inputs(const std::vector<PermInput> &inputs, std::vector<std::vector<int32_t>> &rows,
                   std::vector<int64_t> &item, std::vector<int64_t> &row,
@@ -505,7 +520,7 @@ int fetch_out(Exec &x, std::vector<double> &res) {
   return PQP_OK;
 }
 
-// grad[i, j] = r_i c_j perm(A; r - e_i, c - e_j) from the minor results.
+// grad[i, j] = r_i c_j perm(A; r - e_i, c - e_j) from the column-minor results.
 void scatter_gradients(const Plan &mp, const std::vector<double> &res,
                        const std::vector<int64_t> &mitem, const std::vector<int64_t> &mrow,
                        const std::vector<PermInput> &inputs, double *const *grads) {
@@ -528,6 +543,61 @@ void scatter_gradients(const Plan &mp, const std::vector<double> &res,
     }
   }
 }
+
+// The single-sweep gradient kernels keep up to 32 row sums per lane, and a
+// gradient item's columns of multiplicity c are expanded into c columns, so
+// they take permanents of at most 32 particles; larger ones (up to 64, with
+// at most 32 distinct columns) run as one column-minor sweep per row.
+constexpr int64_t kGradSweepMaxN = 32;
+
+bool gradients_fit_single_sweep(const std::vector<PermInput> &inputs) {
+  for (const PermInput &in : inputs) {
+    int64_t n = 0;
+    for (int64_t i = 0; i < in.m_r; ++i) n += std::max(in.rows[i], 0);
+    if (n > kGradSweepMaxN) return false;
+  }
+  return true;
+}
+
+// Results of a gradient plan: perms[k] (2 doubles, may be null) and
+// grads[k][i, j] = d perm / d A_ij = the sum of G_B[b, j'] over the rows b
+// of B that copy row i of A (the prepended row and row i0's digit row both
+// copy A's row i0) and the kernel columns j' that copy column j (c_j of
+// them); rows or columns of multiplicity 0 get 0.  This equals
+// r_i c_j perm(A; r - e_i, c - e_j) (src/permanent.cpp:271-300).
+void scatter_grad_items(const Plan &p, const std::vector<double> &res,
+                        const std::vector<PermInput> &inputs, double *perms, double *const *grads) {
+  for (size_t k = 0; k < inputs.size(); ++k) {
+    const ItemInfo &inf = p.info[k];
+    const ItemDesc &d = p.items[k];
+    const double *r = res.data() + 2 * inf.out_offset;
+    if (perms) {
+      perms[2 * k] = r[0];
+      perms[2 * k + 1] = r[1];
+    }
+    double *g = grads ? grads[k] : nullptr;
+    if (!g) continue;
+    const PermInput &in = inputs[k];
+    std::memset(g, 0, static_cast<size_t>(in.m_r * in.m_c) * 2 * sizeof(double));
+    if (inf.trivial_one) continue;
+    const int rows_B = 1 + d.L + d.Dt;
+    for (int b = 0; b < rows_B; ++b) {
+      const int64_t i = inf.rowsrc[b];
+      for (int j = 0; j < d.mc; ++j) {
+        // kernel columns have unit multiplicity: a column of multiplicity c
+        // was planned as c copies, whose entries add up here
+        const int64_t col = inf.colsrc[j];
+        const double *v = r + 2 * (1 + static_cast<int64_t>(b) * d.mc + j);
+        g[2 * (i * in.m_c + col)] += v[0];
+        g[2 * (i * in.m_c + col) + 1] += v[1];
+      }
+    }
+  }
+}
+
+// pqp_grad_batch_c128 for permanents above 32 particles: the permanents,
+// then one column-minor sweep per row of non-zero multiplicity.
+int grad_batch_minors(const std::vector<PermInput> &inputs, double *perms, double *const *grads);
 
 }  // namespace
 
@@ -654,12 +724,16 @@ int tensor_plan(int64_t count, const double *const *matrices, const int64_t *n, 
   return PQP_OK;
 }
 
-// One evaluation of a batch: the permanents, then (gradient batches) the
-// column minors on the same stream.
+// One evaluation of a batch (gradient batches have no plain mode: their
+// one sweep gives the permanents and the gradients).
 int enqueue_all(pqp_batch *b, int mode, hipEvent_t ev_begin, hipEvent_t ev_end) {
-  int st = enqueue(b->x, mode, ev_begin, b->grad ? nullptr : ev_end);
-  if (st != PQP_OK || !b->grad) return st;
-  return enqueue(b->xm, mode == PQP_RUN_PLAIN ? PQP_RUN_EAGER : mode, nullptr, ev_end);
+  if (b->grad_minors) {
+    int st = enqueue(b->x, mode, ev_begin, nullptr);
+    if (st != PQP_OK) return st;
+    return enqueue(b->xm, mode == PQP_RUN_PLAIN ? PQP_RUN_EAGER : mode, nullptr, ev_end);
+  }
+  if (b->grad && mode == PQP_RUN_PLAIN) mode = PQP_RUN_EAGER;
+  return enqueue(b->x, mode, ev_begin, ev_end);
 }
 }  // namespace
 
@@ -687,6 +761,60 @@ PolyState &poly_state() {
 }
 
 size_t align256(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
+
+}  // namespace
+
+namespace {
+
+int grad_batch_minors(const std::vector<PermInput> &inputs, double *perms, double *const *grads) {
+  Plan fwd;
+  std::string err;
+  PlanOptions fopt;
+  fopt.total_units = static_cast<int64_t>(std::max(device_state().cus, 1)) * kBatchUnitsPerCu;
+  int st = plan_build(inputs, fopt, fwd, err);
+  if (st != PQP_OK) return fail(st, err);
+  std::vector<std::vector<int32_t>> mrows;
+  std::vector<int64_t> mitem, mrow;
+  std::vector<PermInput> minputs;
+  minor_inputs(inputs, mrows, mitem, mrow, minputs);
+  PlanOptions mopt;
+  mopt.minors = true;
+  mopt.total_units = fopt.total_units;
+  Plan mplan;
+  st = plan_build(minputs, mopt, mplan, err);
+  if (st != PQP_OK) return fail(st, err);
+  if ((st = require_device()) != PQP_OK) return st;
+  SingleState &S = single_state();
+  std::lock_guard<std::mutex> lock(S.mu);
+  Exec &x = S.x;
+  if (!S.init) {
+    PQP_HIP(hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking));
+    x.own_stream = true;
+    S.init = true;
+  }
+  std::vector<double> res;
+  if (perms && !inputs.empty()) {
+    x.plan = std::move(fwd);
+    index_units(x);
+    x.flags = PQP_FLAG_STATIC;
+    if ((st = upload(x)) != PQP_OK) return st;
+    if ((st = enqueue(x, PQP_RUN_EAGER, nullptr, nullptr)) != PQP_OK) return st;
+    if ((st = fetch_out(x, res)) != PQP_OK) return st;
+    std::memcpy(perms, res.data(), inputs.size() * 2 * sizeof(double));
+  }
+  if (!minputs.empty()) {
+    x.plan = std::move(mplan);
+    index_units(x);
+    x.flags = PQP_FLAG_STATIC;
+    if ((st = upload(x)) != PQP_OK) return st;
+    if ((st = enqueue(x, PQP_RUN_EAGER, nullptr, nullptr)) != PQP_OK) return st;
+    if ((st = fetch_out(x, res)) != PQP_OK) return st;
+  } else {
+    x.plan = std::move(mplan);
+  }
+  scatter_gradients(x.plan, res, mitem, mrow, inputs, grads);
+  return PQP_OK;
+}
 
 }  // namespace
 
@@ -1090,23 +1218,15 @@ int pqp_grad_batch_c128(int64_t count, const double *const *matrices, const int6
   std::vector<PermInput> inputs;
   for (int64_t k = 0; k < count; ++k)
     inputs.push_back(PermInput{matrices[k], false, m_r[k], m_c[k], ld[k], rows[k], cols[k]});
-  // validate the permanents themselves (sum check, shapes) before the minors
-  Plan fwd;
+  Plan plan;
   std::string err;
-  PlanOptions fopt;
-  fopt.total_units = static_cast<int64_t>(std::max(device_state().cus, 1)) * kBatchUnitsPerCu;
-  int st = plan_build(inputs, fopt, fwd, err);
+  PlanOptions opt;
+  if (!gradients_fit_single_sweep(inputs)) return grad_batch_minors(inputs, perms, grads);
+  opt.total_units = static_cast<int64_t>(std::max(device_state().cus, 1)) * kGradUnitsPerCu;
+  opt.grad = true;
+  int st = plan_build(inputs, opt, plan, err);
   if (st != PQP_OK) return fail(st, err);
-  std::vector<std::vector<int32_t>> mrows;
-  std::vector<int64_t> mitem, mrow;
-  std::vector<PermInput> minputs;
-  minor_inputs(inputs, mrows, mitem, mrow, minputs);
-  PlanOptions mopt;
-  mopt.minors = true;
-  mopt.total_units = fopt.total_units;
-  Plan mplan;
-  st = plan_build(minputs, mopt, mplan, err);
-  if (st != PQP_OK) return fail(st, err);
+  if (count == 0) return PQP_OK;
   if ((st = require_device()) != PQP_OK) return st;
   SingleState &S = single_state();
   std::lock_guard<std::mutex> lock(S.mu);
@@ -1116,27 +1236,14 @@ int pqp_grad_batch_c128(int64_t count, const double *const *matrices, const int6
     x.own_stream = true;
     S.init = true;
   }
+  x.plan = std::move(plan);
+  index_units(x);
+  x.flags = PQP_FLAG_STATIC;
+  if ((st = upload(x)) != PQP_OK) return st;
+  if ((st = enqueue(x, PQP_RUN_EAGER, nullptr, nullptr)) != PQP_OK) return st;
   std::vector<double> res;
-  if (perms && count > 0) {
-    x.plan = std::move(fwd);
-    index_units(x);
-    x.flags = PQP_FLAG_STATIC;
-    if ((st = upload(x)) != PQP_OK) return st;
-    if ((st = enqueue(x, PQP_RUN_EAGER, nullptr, nullptr)) != PQP_OK) return st;
-    if ((st = fetch_out(x, res)) != PQP_OK) return st;
-    std::memcpy(perms, res.data(), static_cast<size_t>(count) * 2 * sizeof(double));
-  }
-  if (!minputs.empty()) {
-    x.plan = std::move(mplan);
-    index_units(x);
-    x.flags = PQP_FLAG_STATIC;
-    if ((st = upload(x)) != PQP_OK) return st;
-    if ((st = enqueue(x, PQP_RUN_EAGER, nullptr, nullptr)) != PQP_OK) return st;
-    if ((st = fetch_out(x, res)) != PQP_OK) return st;
-    scatter_gradients(x.plan, res, mitem, mrow, inputs, grads);
-  } else {
-    scatter_gradients(x.plan, res, mitem, mrow, inputs, grads);
-  }
+  if ((st = fetch_out(x, res)) != PQP_OK) return st;
+  scatter_grad_items(x.plan, res, inputs, perms, grads);
   return PQP_OK;
 }
 
@@ -1166,8 +1273,6 @@ int pqp_batch_create(int64_t count, const void *const *matrices, int is_c64, con
                         ? total_units
                         : static_cast<int64_t>(std::max(device_state().cus, 1)) * kBatchUnitsPerCu;
   std::string err;
-  int st = plan_build(inputs, opt, b->x.plan, err);
-  if (st != PQP_OK) return fail(st, err);
   if (flags & PQP_FLAG_GRADIENT) {
     if (is_c64) return fail(PQP_ERR_INVALID, "gradients need complex128 input");
     b->grad = true;
@@ -1179,6 +1284,14 @@ int pqp_batch_create(int64_t count, const void *const *matrices, int is_c64, con
       inputs[k].rows = b->rows.back().data();
       inputs[k].cols = b->cols.back().data();
     }
+    b->grad_minors = !gradients_fit_single_sweep(inputs);
+    opt.grad = !b->grad_minors;
+    if (opt.grad && total_units <= 0)
+      opt.total_units = static_cast<int64_t>(std::max(device_state().cus, 1)) * kGradUnitsPerCu;
+  }
+  int st = plan_build(inputs, opt, b->x.plan, err);
+  if (st != PQP_OK) return fail(st, err);
+  if (b->grad_minors) {
     std::vector<PermInput> minputs;
     minor_inputs(inputs, b->minor_rows, b->minor_item, b->minor_row, minputs);
     PlanOptions mopt;
@@ -1193,7 +1306,7 @@ int pqp_batch_create(int64_t count, const void *const *matrices, int is_c64, con
   PQP_HIP(hipStreamCreateWithFlags(&b->x.stream, hipStreamNonBlocking));
   b->x.own_stream = true;
   if ((st = upload(b->x)) != PQP_OK) return st;
-  if (b->grad) {
+  if (b->grad_minors) {
     index_units(b->xm);
     b->xm.flags = PQP_FLAG_STATIC;
     b->xm.stream = b->x.stream;  // shares the batch stream (and its graph)
@@ -1246,6 +1359,20 @@ int pqp_batch_results(pqp_batch *b, double *out, double *out_dd) {
   if (!b->ran) return fail(PQP_ERR_INVALID, "batch has not been run");
   const size_t n = b->x.plan.items.size();
   PQP_HIP(hipStreamSynchronize(b->x.stream));
+  if (b->grad && !b->grad_minors && n) {
+    // gradient plans: the permanent of item k sits at its first output slot
+    std::vector<double> all(2 * b->x.n_results), all_dd(4 * b->x.n_results);
+    PQP_HIP(hipMemcpy(all.data(), b->x.results.p, all.size() * sizeof(double), hipMemcpyDeviceToHost));
+    if (out_dd)
+      PQP_HIP(hipMemcpy(all_dd.data(), static_cast<double *>(b->x.results.p) + 2 * b->x.n_results,
+                        all_dd.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < n; ++k) {
+      const int64_t o = b->x.plan.info[k].out_offset;
+      if (out) std::memcpy(out + 2 * k, all.data() + 2 * o, 2 * sizeof(double));
+      if (out_dd) std::memcpy(out_dd + 4 * k, all_dd.data() + 4 * o, 4 * sizeof(double));
+    }
+    return PQP_OK;
+  }
   if (out && n) PQP_HIP(hipMemcpy(out, b->x.results.p, n * 2 * sizeof(double), hipMemcpyDeviceToHost));
   if (out_dd && n)
     PQP_HIP(hipMemcpy(out_dd, static_cast<double *>(b->x.results.p) + 2 * b->x.n_results,
@@ -1288,19 +1415,18 @@ int pqp_batch_time(pqp_batch *b, int mode, int reps, float *ms) {
 int pqp_batch_stats(const pqp_batch *b, double *flops, int64_t *terms, int64_t *units,
                     int64_t *arena_bytes, int64_t *lane_terms) {
   if (!b) return fail(PQP_ERR_INVALID, "batch is null");
+  const bool m = b->grad_minors;
   if (lane_terms) {
     int64_t lt = 0;
     for (const Plan *p : {&b->x.plan, &b->xm.plan})
       for (const ItemDesc &d : p->items) lt += static_cast<int64_t>(d.units) * kWave * d.S_in;
     *lane_terms = lt;
   }
-  if (flops) *flops = b->x.plan.total_flops + (b->grad ? b->xm.plan.total_flops : 0.0);
-  if (terms) *terms = b->x.plan.total_terms + (b->grad ? b->xm.plan.total_terms : 0);
-  if (units)
-    *units = static_cast<int64_t>(b->x.plan.units.size() + (b->grad ? b->xm.plan.units.size() : 0));
+  if (flops) *flops = b->x.plan.total_flops + (m ? b->xm.plan.total_flops : 0.0);
+  if (terms) *terms = b->x.plan.total_terms + (m ? b->xm.plan.total_terms : 0);
+  if (units) *units = static_cast<int64_t>(b->x.plan.units.size() + (m ? b->xm.plan.units.size() : 0));
   if (arena_bytes)
-    *arena_bytes =
-        static_cast<int64_t>(b->x.plan.arena.size() + (b->grad ? b->xm.plan.arena.size() : 0));
+    *arena_bytes = static_cast<int64_t>(b->x.plan.arena.size() + (m ? b->xm.plan.arena.size() : 0));
   return PQP_OK;
 }
 
@@ -1311,12 +1437,17 @@ int pqp_batch_gradients(pqp_batch *b, double *const *grads) {
   if (!b->ran) return fail(PQP_ERR_INVALID, "batch has not been run");
   std::vector<double> res;
   int st;
-  if (!b->minor_item.empty() && (st = fetch_out(b->xm, res)) != PQP_OK) return st;
   std::vector<PermInput> inputs;
   for (size_t k = 0; k < b->m_r.size(); ++k)
     inputs.push_back(PermInput{nullptr, false, b->m_r[k], b->m_c[k], b->m_c[k],
                                b->rows[k].data(), b->cols[k].data()});
-  scatter_gradients(b->xm.plan, res, b->minor_item, b->minor_row, inputs, grads);
+  if (b->grad_minors) {
+    if (!b->minor_item.empty() && (st = fetch_out(b->xm, res)) != PQP_OK) return st;
+    scatter_gradients(b->xm.plan, res, b->minor_item, b->minor_row, inputs, grads);
+    return PQP_OK;
+  }
+  if ((st = fetch_out(b->x, res)) != PQP_OK) return st;
+  scatter_grad_items(b->x.plan, res, inputs, nullptr, grads);
   return PQP_OK;
 }
 
@@ -1331,6 +1462,7 @@ int pqp_plan_image(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
   opt.target_inner = target_inner;
   opt.min_units = min_units;
   opt.minors = (flags & PQP_FLAG_GRADIENT) != 0;
+  opt.grad = !opt.minors && (flags & PQP_PLAN_GRADIENT_SWEEP) != 0;
   Plan p;
   std::string err;
   const int st = plan_build(std::vector<PermInput>{in}, opt, p, err);

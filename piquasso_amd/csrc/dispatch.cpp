@@ -48,6 +48,18 @@ hipError_t occupancy_fwd(int bucket, bool persistent, bool fp32, bool comp, int 
 }
 #undef PQP_CASE
 
+hipError_t launch_grad(int bucket, const LaunchArgs &a, int first_unit, int n_units,
+                       hipStream_t stream) {
+  switch (bucket) {
+    case 0: return launch_grad_b0(a, first_unit, n_units, stream);
+    case 1: return launch_grad_b1(a, first_unit, n_units, stream);
+    case 2: return launch_grad_b2(a, first_unit, n_units, stream);
+    case 3: return launch_grad_b3(a, first_unit, n_units, stream);
+    case 4: return launch_grad_b4(a, first_unit, n_units, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 hipError_t launch_minors(int bucket, const LaunchArgs &a, int first_unit, int n_units,
                          hipStream_t stream) {
   switch (bucket) {

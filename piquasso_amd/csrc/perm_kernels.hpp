@@ -60,6 +60,21 @@ hipError_t launch_minors(int bucket, const LaunchArgs &a, int first_unit, int n_
 hipError_t launch_minors_finalize(const LaunchArgs &a, int n_items, int max_units,
                                   hipStream_t stream);
 
+// Gradient kernels (perm_grad.hip), unit column multiplicities: buckets for
+// widths 1-8, 9-16, 17-19, 20, 21-32.
+constexpr int kGradBuckets = 5;
+hipError_t launch_grad_b0(const LaunchArgs &a, int first_unit, int n_units, hipStream_t s);
+hipError_t launch_grad_b1(const LaunchArgs &a, int first_unit, int n_units, hipStream_t s);
+hipError_t launch_grad_b2(const LaunchArgs &a, int first_unit, int n_units, hipStream_t s);
+hipError_t launch_grad_b3(const LaunchArgs &a, int first_unit, int n_units, hipStream_t s);
+hipError_t launch_grad_b4(const LaunchArgs &a, int first_unit, int n_units, hipStream_t s);
+hipError_t launch_grad(int bucket, const LaunchArgs &a, int first_unit, int n_units,
+                       hipStream_t stream);
+// max_values: the largest 1 + rows of B x m_c over the items (the
+// permanent and its gradient entries)
+hipError_t launch_grad_finalize(const LaunchArgs &a, int n_items, int max_values,
+                                hipStream_t stream);
+
 // Tensor permanents (tensor_perm.hip): one launch per matrix order n over
 // units [first_unit, first_unit + n_units), then one finalize for all items.
 hipError_t launch_tensor_p0(int n, const TensorArgs &a, int first_unit, int n_units, hipStream_t s);

@@ -12,6 +12,7 @@ namespace {
 
 constexpr int64_t kMaxTerms = int64_t(1) << 62;
 constexpr int64_t kMaxInner = int64_t(1) << 16;   // step/coef table length cap
+constexpr int64_t kGradMaxStates = 128;           // inner states per lane of a gradient unit
 constexpr int kExpandAutoMax = 0;                 // auto mode: never (measured slower)
 
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
@@ -155,7 +156,9 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
       const int rho = in.rows[i] - (i == i0 ? 1 : 0);
       if (rho > 0) T *= rho + 1;
     }
-    item_cost[k] = opt.minors ? T * (16.0 * mc + 10.0) : T * (2.0 * mc + 4.0 * (sr - 1) + 8.0);
+    item_cost[k] = opt.minors ? T * (16.0 * mc + 10.0)
+                   : opt.grad ? T * (25.0 * mc + 10.0)
+                              : T * (2.0 * mc + 4.0 * (sr - 1) + 8.0);
     item_terms[k] = T;
     total_cost += item_cost[k];
   }
@@ -222,6 +225,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
       d.units = 0;
       d.n = 0;
       d.partial_base = partial_cursor;
+      if (opt.grad) d.flags = kGrad;
       if (opt.minors) {
         d.flags = kMinors;
         plan.info[k].colsrc.reserve(static_cast<size_t>(in.m_c));
@@ -256,10 +260,14 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     // wave-uniform branches), or is expanded into c identical columns of
     // multiplicity 1 (branch-free product, 2 (c - 1) more additions per term).
     bool expand = false;
-    if (cmax_in > 1 && !opt.minors) {
+    if (cmax_in > 1 && !opt.minors && !opt.grad) {
       if (opt.expand_cols == 1) expand = n <= kMaxCols;
       if (opt.expand_cols == 2) expand = n <= kExpandAutoMax;
     }
+    // gradient items: the kernel works with unit column multiplicities; a
+    // column of multiplicity c becomes c identical columns, whose gradient
+    // entries add up to the column's (its c_j factor)
+    if (cmax_in > 1 && opt.grad) expand = true;
     colsrc.clear();   // source column of each kernel column
     colmult.clear();  // multiplicity of each kernel column
     for (int64_t j : colsel) {
@@ -278,6 +286,10 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     const int bucket = bucket_for_width(mc, opt.fp32_products, &mcp);
     if (opt.minors && mcp > 32) {
       err = "item " + std::to_string(k) + ": column minors support at most 32 columns";
+      return kErrRange;
+    }
+    if (opt.grad && mcp > 32) {
+      err = "item " + std::to_string(k) + ": single-sweep gradients support at most 32 particles";
       return kErrRange;
     }
     if (bucket < 0) {
@@ -324,17 +336,26 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     for (int c : colmult) cmax = std::max(cmax, c);
     const double step_cost =
         opt.minors ? 16.0 * mc + 10.0 + (cmax > 1 ? 8.0 * (n + 1 - mc) : 0.0)
+        : opt.grad ? 25.0 * mc + 10.0 + (cmax > 1 ? 8.0 * (n - mc) : 0.0)
                    : 2.0 * mc + 4.0 * (n - 1) + 2.0 + 6.0 + (cmax > 1 ? 2.0 * (n - mc) : 0.0);
     // per-unit overhead: start state, plus the wave reduction of every
-    // column accumulator for column-minor items
-    const double unit_extra = opt.minors ? 160.0 * (mc + 1) : 0.0;
+    // column accumulator for column-minor items, and for gradient items the
+    // lane reduction of each top row and the writes of every row
+    const double unit_extra = opt.minors ? 160.0 * (mc + 1)
+                              : opt.grad ? 12.0 * mc * D + 8.0 * (D + 1) + 200.0
+                                         : 0.0;
     int best_L = 0;
     double best_total = -1.0;
+    // gradient items: a work unit is one wave sweeping S_in states, and the
+    // batch's time is at least its longest unit's, so S_in is capped
+    // (target_inner, else kGradMaxStates) and the cost model chooses below it
+    const int64_t s_cap = opt.grad ? (opt.target_inner > 0 ? opt.target_inner : kGradMaxStates)
+                                   : kMaxInner;
     {
       int64_t S = 1;
       for (int L = 0; L <= D; ++L) {
         if (L > 0) S *= (digits[order[L - 1]].rho + 1);
-        if (S > kMaxInner) break;
+        if (S > kMaxInner || (L > 0 && S > s_cap)) break;
         const int64_t U = T / S;
         const int64_t units = (U + kWave - 1) / kWave;
         const int Dt = D - L;
@@ -342,7 +363,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
         const double unit_cost = init_cost + static_cast<double>(S) * step_cost;
         const double total = static_cast<double>(units) * unit_cost;
         bool ok = true;
-        if (opt.target_inner > 0) {
+        if (opt.target_inner > 0 && !opt.grad) {
           // largest inner size not above the request
           ok = (S <= opt.target_inner) || L == 0;
           if (ok) {
@@ -418,10 +439,22 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
         put(d.b_off, r, j, re, im);
       }
     }
-    // inner start coefficients alpha_k = rho_k (all inner g = 0)
-    d.alpha_off = arena_alloc(plan.arena, int64_t(std::max(L, 1)) * 8);
+    // inner start coefficients alpha_k = rho_k (all inner g = 0); gradient
+    // items also keep the coefficients of the last inner state: the counter
+    // ends at all digits radix - 1, whose reflected Gray digits are, top
+    // down, radix - 1 or 0 by the parity of the higher Gray digits
+    d.alpha_off = arena_alloc(plan.arena, int64_t(std::max(L, 1)) * (opt.grad ? 16 : 8));
     for (int t = 0; t < L; ++t)
       at<double>(plan.arena, d.alpha_off)[t] = static_cast<double>(digits[inner[t]].rho);
+    if (opt.grad) {
+      int parity = 0;
+      for (int t = L - 1; t >= 0; --t) {
+        const int rho = digits[inner[t]].rho;
+        const int g = parity ? 0 : rho;
+        parity ^= (g & 1);
+        at<double>(plan.arena, d.alpha_off)[L + t] = static_cast<double>(rho - 2 * g);
+      }
+    }
     // shared inner step / coefficient tables
     profile.clear();
     for (int x : inner) profile.push_back(digits[x].rho);
@@ -515,11 +548,17 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     d.Dt = Dt;
     d.cmax = cmax;
     d.flags = (cmax == 1 ? kColsAllOnes : 0) | (top_binary ? kTopBinary : 0) |
-              (opt.minors ? kMinors : 0) | (grouped ? kPowGrouped : 0) |
-              (opt.direct_out && !opt.minors && d.units == 1 ? kDirectOut : 0);
+              (opt.minors ? kMinors : 0) | (opt.grad ? kGrad : 0) | (grouped ? kPowGrouped : 0) |
+              (opt.direct_out && !opt.minors && !opt.grad && d.units == 1 ? kDirectOut : 0);
     d.partial_base = partial_cursor;
-    partial_cursor += static_cast<int64_t>(d.units) * (opt.minors ? mcp + 1 : 1);
-    out_cursor += opt.minors ? mc + 1 : 1;
+    if (opt.grad) {
+      partial_cursor += static_cast<int64_t>(d.units) * (grad_unit_stride(rows_B, mcp) / 4);
+      out_cursor += 1 + static_cast<int64_t>(rows_B) * mc;
+      plan.info[k].rowsrc.assign(src_rows.begin(), src_rows.end());
+    } else {
+      partial_cursor += static_cast<int64_t>(d.units) * (opt.minors ? mcp + 1 : 1);
+      out_cursor += opt.minors ? mc + 1 : 1;
+    }
     plan.info[k].colsrc.assign(colsrc.begin(), colsrc.end());
     plan.items[k] = d;
 
@@ -527,11 +566,14 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     inf.terms = T;
     // minors: row update, prefix / suffix / combine products (3 (m_c - 1)
     // complex multiplies, extra powers 2 (n + 1 - m_c)), m_c + 1 accumulations
+    // gradients: row update 2 m_c, exclusion products 18 (n - 1), the
+    // accumulation 2 m_c, one row flush 4 m_c, the weight 4 per term
     inf.flops = opt.minors
                     ? static_cast<double>(T) * (2.0 * mc_alg + 6.0 * (3.0 * (mc_alg - 1) +
                                                                    2.0 * (n + 1 - mc_alg)) +
                                                 4.0 * (mc_alg + 1))
-                    : static_cast<double>(T) * (2.0 * mc_alg + 6.0 * (n - 1) + 4.0);
+                : opt.grad ? static_cast<double>(T) * (8.0 * mc_alg + 18.0 * (n - 1) + 4.0)
+                           : static_cast<double>(T) * (2.0 * mc_alg + 6.0 * (n - 1) + 4.0);
     inf.unit_cost = ((1.0 + D) * 2.0 * mcp + Dt * 40.0 + 400.0) +
                     static_cast<double>(S_in) * step_cost;
     plan.total_flops += inf.flops;
@@ -543,6 +585,8 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     if (opt.minors)
       item_bucket[k] = cmax > 1 ? bucket + 5
                                 : (mc <= 8 ? 0 : mc <= 16 ? 1 : mc <= 18 ? 2 : mc <= 20 ? 3 : 4);
+    else if (opt.grad)
+      item_bucket[k] = mc <= 8 ? 0 : mc <= 16 ? 1 : mc <= 19 ? 2 : mc == 20 ? 3 : 4;
     else
       item_bucket[k] = bucket;
   }
@@ -554,6 +598,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
   }
   plan.n_out = out_cursor;
   plan.minors = opt.minors;
+  plan.grad = opt.grad;
 
   // --- work units: grouped by kernel bucket, LPT (costliest units first) ---
   std::vector<int64_t> idx(n_items);

@@ -51,6 +51,9 @@ struct PlanOptions {
                                 // uncut product (m_c + 1 values per item)
   int64_t total_units = 0;      // spread about this many waves over the batch, in
                                 // proportion to each permanent's cost (0 = off)
+  bool grad = false;            // gradient items: the permanent and d perm / d B for every
+                                // row of B from one sweep (perm_grad.hip); outputs
+                                // 1 + rows_B * m_c values per item
 };
 
 // Per-item bookkeeping kept on the host.
@@ -61,6 +64,7 @@ struct ItemInfo {
   bool trivial_one = false;     // n == 0: permanent is 1 by definition
   int64_t out_offset = 0;       // first output slot (complex values)
   std::vector<int64_t> colsrc;  // source column of each kernel column
+  std::vector<int64_t> rowsrc;  // gradient items: source row of each row of B
 };
 
 struct Plan {
@@ -79,6 +83,7 @@ struct Plan {
   bool all_direct = false;    // every item writes its own result: no finalize launch
   int64_t n_out = 0;          // output slots (complex values)
   bool minors = false;        // column-minor items (perm_minors.hip kernels)
+  bool grad = false;          // gradient items (perm_grad.hip kernels)
   bool fp32 = false;
   double total_flops = 0.0;
   int64_t total_terms = 0;

@@ -34,7 +34,18 @@ enum ItemFlags : int {
   kMinors = 4,        // column-minor item: perm(A; r, c - e_l) for every column l
   kPowGrouped = 8,    // column powers from prefix-product snapshots (masks at cpow_off)
   kDirectOut = 16,    // one work unit: the unit writes the scaled result itself (no finalize)
+  kGrad = 32,         // gradient item: the permanent and d perm / d B for every row of B
 };
+
+constexpr int kGradMaxInner = 16;  // inner digits of a gradient item (per-row LDS accumulators)
+
+// Doubles per work unit of a gradient item: the permanent's dd partial (4),
+// then one FP64 value per (row of B, real component) with the components of
+// a row laid out [re_0..re_{mcp-1}, im_0..im_{mcp-1}]; rounded up to whole
+// 4-double partial slots.  (constexpr: usable in host and device code.)
+constexpr int64_t grad_unit_stride(int rows_B, int mcp) {
+  return (4 + static_cast<int64_t>(rows_B) * 2 * mcp + 3) / 4 * 4;
+}
 
 // One permanent; read by the kernels with scalar loads (the item index is
 // wave-uniform).

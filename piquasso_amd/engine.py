@@ -145,7 +145,11 @@ class PermanentBatch:
             raise ValueError("precision must be 'fp64' or 'fp32'")
         self.count = len(matrices)
         self.precision = precision
-        if persistent is None or not total_units:
+        if gradient:
+            # gradient sweeps: one block per work unit, the library's unit count
+            if persistent is None:
+                persistent = False
+        elif persistent is None or not total_units:
             from .tune import best_config
 
             try:

@@ -64,8 +64,9 @@ def test_each_allocation_failure_is_clean(gpu, oracle, gradient):
         break
     else:
         pytest.fail("batch creation never succeeded")
-    # arena, partials, results, queue heads, pinned stage (and the minors' set)
-    assert failures >= (10 if gradient else 5)
+    # arena, partials, results, queue heads, pinned stage (a gradient batch
+    # is one plan too: its sweep gives the permanents and the gradients)
+    assert failures >= 5
     # the hook is off again and the library still computes
     b = PermanentBatch(list(mats), list(rows), list(cols))
     assert np.max(np.abs(b.run("graph").results() - want) / np.abs(want)) < 1e-12

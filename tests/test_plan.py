@@ -269,3 +269,132 @@ def test_grouped_power_masks():
     buf = plan_image(a, [4, 3, 2], [6, 1, 1, 1])
     d = np.frombuffer(buf[:128].tobytes(), dtype=ITEM)[0]
     assert not d["flags"] & 8  # drop of 5 > 3: count form
+
+
+def emulate_gradient(buf, a, r, c):
+    """Gradient plans (pqp_grad_batch_c128): the arithmetic of perm_grad.hip on
+    the planned arena, unit by unit.  Per term the exclusion products
+    Y_j = s c_j cs_j^(c_j - 1) prod_(k != j) cs_k^(c_k); the prepended row's
+    gradient is the sum of Y over lanes and terms; an inner row's comes from
+    the running wave sum V and its accumulator K (K -= (coefficient change) V
+    after each term in which the row moves, then K + final coefficient x V);
+    a top row's from its lane-constant coefficient times the lane's sum of Y.
+    The rows of B are mapped back to rows of A by value (A's rows are
+    distinct here), the prepended row and row i0's digit row both to i0."""
+    d = np.frombuffer(buf[:128].tobytes(), dtype=ITEM)[0]
+    assert d["flags"] & 32
+    mc, mcp, L, Dt, S, U = (int(d[k]) for k in ("mc", "mcp", "L", "Dt", "S_in", "U"))
+
+    def f64(off, count):
+        return np.frombuffer(buf[off: off + 8 * count].tobytes(), dtype=np.float64)
+
+    B = f64(int(d["b_off"]), (1 + L + Dt) * mcp * 2).view(np.complex128).reshape(1 + L + Dt, mcp)
+    alpha = f64(int(d["alpha_off"]), 2 * max(L, 1))
+    a_start, a_final = alpha[:L], alpha[L: 2 * L]
+    coef, steps = inner_table(buf, d, S)
+    top = f64(int(d["top_off"]), 4 * max(Dt, 1)).reshape(-1, 4)
+    cm = np.frombuffer(buf[int(d["cmul_off"]): int(d["cmul_off"]) + 4 * mc].tobytes(), dtype=np.int32)
+    rows_B = 1 + L + Dt
+    GB = np.zeros((rows_B, mc), dtype=np.complex128)
+    perm = 0j
+    for unit in range(int(d["units"])):
+        lanes = [u for u in range(64 * unit, 64 * unit + 64) if u < U]
+        cs, lw, tal = {}, {}, {}
+        for u in lanes:
+            x = B[0].copy()
+            for k in range(L):
+                x = x + a_start[k] * B[1 + k]
+            w, parity, q_hi = 1.0, 0, 0.0
+            al = np.zeros(Dt)
+            for t in range(Dt - 1, -1, -1):
+                inv_place, radix, rho = top[t, 0], top[t, 1], int(top[t, 2])
+                q = math.floor((u + 0.5) * inv_place)
+                cdig = int(q - radix * q_hi)
+                q_hi = q
+                g = int(radix) - 1 - cdig if parity else cdig
+                parity ^= g & 1
+                w *= math.comb(rho, g)
+                al[t] = rho - 2 * g
+                x = x + al[t] * B[1 + L + t]
+            cs[u], lw[u], tal[u] = x, (-w if (u & 1) else w), al
+        W = {u: np.zeros(mc, dtype=np.complex128) for u in lanes}
+        V = np.zeros(mc, dtype=np.complex128)
+        K = np.zeros((max(L, 1), mc), dtype=np.complex128)
+        for v in range(S):
+            Yw = np.zeros(mc, dtype=np.complex128)
+            for u in lanes:
+                x = cs[u][:mc]
+                cf = coef[v] * lw[u]
+                y = np.array([cf * cm[j] * x[j] ** (cm[j] - 1) *
+                              np.prod([x[k] ** cm[k] for k in range(mc) if k != j]) for j in range(mc)])
+                perm += cf * np.prod(x ** cm)
+                W[u] += y
+                Yw += y
+            V += Yw
+            if v + 1 < S:
+                s2, row = float(steps[v]["sign2"]), int(steps[v]["row"])
+                K[row] -= s2 * V
+                for u in lanes:
+                    cs[u] = cs[u] + s2 * B[1 + row]
+        GB[0] += V
+        for i in range(L):
+            GB[1 + i] += K[i] + a_final[i] * V
+        for t in range(Dt):
+            GB[1 + L + t] += sum(tal[u][t] * W[u] for u in lanes)
+    scale = 2.0 ** -(int(d["n"]) - 1)
+    perm *= scale
+    GB *= scale
+    # back to A's rows and columns: kernel columns are the non-zero columns
+    # stably sorted by decreasing multiplicity, a column of multiplicity c
+    # expanded into c copies whose entries add up
+    order = sorted(np.flatnonzero(np.asarray(c) > 0), key=lambda j: -c[j])
+    kcols = [j for j in order for _ in range(int(c[j]))]
+    assert len(kcols) == mc and (cm == 1).all()
+    G = np.zeros(a.shape, dtype=np.complex128)
+    for b in range(rows_B):
+        i = int(np.argmin(np.abs(a[:, kcols] - B[b, :mc]).sum(axis=1)))
+        assert np.allclose(a[i, kcols], B[b, :mc])
+        for jj, j in enumerate(kcols):
+            G[i, j] += GB[b, jj]
+    return perm, G
+
+
+def test_gradient_plan_emulation_matches_oracle(oracle):
+    """The single-sweep gradient arena (the forward tables plus the inner rows'
+    final coefficients) reproduces the explicit minors r_i c_j perm(A; r - e_i,
+    c - e_j), with inner and top digits of higher radix (row multiplicities)
+    and columns of multiplicity c planned as c unit columns."""
+    rng = np.random.default_rng(91)
+    for trial in range(30):
+        m_r, m_c = (int(x) for x in rng.integers(1, 6, size=2))
+        n = int(rng.integers(1, 8))
+        a = rng.standard_normal((m_r, m_c)) + 1j * rng.standard_normal((m_r, m_c))
+        r = np.bincount(rng.integers(0, m_r, n), minlength=m_r)
+        c = np.bincount(rng.integers(0, m_c, n), minlength=m_c)
+        ref = oracle.grad_dd(a, r, c)
+        p_ref = oracle.perm_dd(a, r, c)
+        for target in (0, 1, 2, 4):
+            buf = plan_image(a, r, c, target_inner=target, flags=_capi.PQP_PLAN_GRADIENT_SWEEP)
+            p, g = emulate_gradient(buf, a, r, c)
+            scale = max(np.abs(ref).max(), 1e-300)
+            assert abs(p - p_ref) <= 1e-11 * max(abs(p_ref), 1e-300), (trial, target)
+            assert np.abs(g - ref).max() <= 1e-11 * scale, (trial, target, g, ref)
+
+
+def test_gradient_plan_final_coefficients():
+    """The last inner state of the reflected Gray code: with all counter digits
+    at radix - 1, a digit is at its top value when the higher Gray digits
+    have even sum, else at 0 (rows with multiplicities 3, 1, 2 as inner)."""
+    a = np.arange(1, 13, dtype=float).reshape(4, 3) + 0j
+    r = [1, 3, 1, 2]
+    buf = plan_image(a, r, [3, 2, 2], target_inner=1 << 10, flags=_capi.PQP_PLAN_GRADIENT_SWEEP)
+    d = np.frombuffer(buf[:128].tobytes(), dtype=ITEM)[0]
+    L = int(d["L"])
+    alpha = np.frombuffer(buf[int(d["alpha_off"]): int(d["alpha_off"]) + 16 * L].tobytes(), np.float64)
+    rho = alpha[:L].astype(int)
+    # walk the Gray sequence: the final state's coefficients
+    coef, steps = inner_table(buf, d, int(d["S_in"]))
+    g = np.zeros(L, int)
+    for s in steps:
+        g[int(s["row"])] += 1 if s["sign2"] < 0 else -1
+    assert (alpha[L:] == rho - 2 * g).all()
