@@ -777,9 +777,9 @@ def main():
         "output_probabilities": probabilities,
         "single_permanents": singles,
         "roofline": {
-            "bound": "mfma",
-            "pipe": "FP64 VALU: the sweep is a product of row sums, not a contraction; "
-                    "MI355X's FP64 vector and FP64 matrix peaks are both 78.6 TFLOP/s",
+            "bound": "fp64-valu",
+            "pipe": "FP64 vector ALU (no MFMA: the sweep is a product of row sums, not a "
+                    "contraction); the peak is MI355X's FP64 vector peak, 78.6 TFLOP/s",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",

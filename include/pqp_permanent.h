@@ -84,6 +84,15 @@ int pqp_debug_fail_allocation(int64_t after);
  *   out    : 2 doubles (re, im), the double-double result rounded to FP64
  *   out_dd : 4 doubles (re_hi, re_lo, im_hi, im_lo) or null
  *   flags  : 0, PQP_FLAG_PLAIN or PQP_FLAG_FP32
+ * Integer matrices: without PQP_FLAG_PLAIN, a matrix whose selected entries
+ * are integers (|a| <= 2^47, zero imaginary parts) and whose Glynn terms may
+ * reach 2^47 (2^24 with PQP_FLAG_FP32) is evaluated by the exact
+ * multi-modular path of pqp_permanent_int64: `out` is the exact permanent
+ * rounded once to nearest, `out_dd` its two leading doubles.  Below that
+ * bound the floating-point sweep is exact already: every product is an
+ * exact integer, runs of 64 of them are summed in FP64 below 2^53, and the
+ * double-double sums of integers below 2^100 are exact.  Inputs the exact
+ * path refuses (PQP_ERR_RANGE) take the floating-point sweep.
  */
 int pqp_permanent_c128(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
                        const int32_t *rows, const int32_t *cols, uint32_t flags,
@@ -94,7 +103,8 @@ int pqp_permanent_c128(const double *matrix, int64_t m_r, int64_t m_c, int64_t l
  * host allocation in the call.  No counterpart in the reference: this entry
  * is for callers that cannot allocate device memory (serving loops and
  * harnesses that forbid it).  Same arguments, flags and result as
- * pqp_permanent_c128, from the same kernels and plan (bit-identical).  The
+ * pqp_permanent_c128, from the same kernels and plan (bit-identical),
+ * except that integer matrices are not routed to the exact path.  The
  * device memory the call needs (plan tables, per-wave partial sums, result
  * slots) comes from a fixed 8 MiB device array that each GPU holds for the
  * life of the process; that holds permanents of up to 2^39 Gray terms
@@ -119,6 +129,27 @@ int pqp_permanent_c128_noalloc(const double *matrix, int64_t m_r, int64_t m_c, i
 int pqp_permanent_c64(const float *matrix, int64_t m_r, int64_t m_c, int64_t ld,
                       const int32_t *rows, const int32_t *cols, uint32_t flags, float *out,
                       double *out_dd);
+
+/*
+ * The exact permanent of an integer matrix with multiplicities.
+ * Replaces permanent_cpp<T> (src/permanent.cpp:49-264) for integer-valued
+ * input such as the reference's integer case
+ * (tests/_math/test_permanent.py:252-282): the same Glynn sum over the same
+ * Gray order, taken modulo up to eight primes just below 2^49 on the GPU
+ * (FP64 modular multiplies), the integer recovered on the host by the
+ * Chinese remainder theorem.  Counts perfect matchings of 0/1 matrices
+ * exactly at every size the forward path takes (n <= 64).
+ *   matrix : m_r x m_c int64, row-major, leading dimension ld
+ *   out    : 1 double, the permanent rounded once to nearest (ties to even)
+ *   limbs  : n_limbs >= 8 uint64 words receiving the permanent as a 512-bit
+ *            little-endian two's complement integer, or null
+ * PQP_ERR_RANGE when a selected entry exceeds 2^47 in magnitude, a
+ * column's sum_i r_i |a_ij| exceeds 2^48, prod_i C(r_i, r_i / 2) exceeds
+ * 2^52, or the bound prod_i (sum_j c_j |a_ij|)^{r_i} exceeds 2^390.
+ */
+int pqp_permanent_int64(const int64_t *matrix, int64_t m_r, int64_t m_c, int64_t ld,
+                        const int32_t *rows, const int32_t *cols, double *out, uint64_t *limbs,
+                        int32_t n_limbs);
 
 /*
  * Batches: many permanents of uneven size, one launch per column-width

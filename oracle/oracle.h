@@ -70,6 +70,14 @@ int pqo_laplace_dd_c128(const double *a, int m_r, int m_c, const int *rows,
 int pqo_gray_states(const int *limits, int n_digits, int64_t offset,
                     int64_t count, int *codes);
 
+/* Exact permanent of an integer matrix (row-major int64, leading dimension
+ * m_c; entries and row sums below 2^53): the same Glynn sum in exact
+ * multi-word integer arithmetic.  limbs: 8 x 64-bit, little-endian two's
+ * complement (512 bits).  Returns 0, -1 (sum mismatch), -2 (invalid
+ * arguments), -3 (a term leaves 511 bits), -4 (internal check). */
+int pqo_perm_int_exact(const int64_t *a, int m_r, int m_c, const int *rows, const int *cols,
+                       int n_jobs, uint64_t *limbs);
+
 /* Number of processors OpenMP sees (used for the default chunking). */
 int pqo_num_procs(void);
 

@@ -88,6 +88,8 @@ def lib():
             ("pqo_gray_states", [ip, ci, ctypes.c_int64, ctypes.c_int64, ip]),
             ("pqo_tensor_perm_dd", [dp, ci, ctypes.c_double, ci, dp, dp]),
             ("pqo_perm_poly_dd", [dp, dp, ci, ci, ctypes.POINTER(ctypes.c_int32), ci, dp, dp]),
+            ("pqo_perm_int_exact", [ctypes.POINTER(ctypes.c_int64), ci, ci, ip, ip, ci,
+                                    ctypes.POINTER(ctypes.c_uint64)]),
         ):
             fn = getattr(L, name)
             fn.argtypes = argt
@@ -166,6 +168,27 @@ def perm_dd(matrix, rows, cols, n_jobs: int = 0) -> complex:
     """Extended-precision permanent rounded to complex128."""
     p = perm_dd_parts(matrix, rows, cols, n_jobs)
     return complex(p[0] + p[1], p[2] + p[3])
+
+
+def perm_int_exact(matrix, rows, cols, n_jobs: int = 0) -> int:
+    """Exact permanent of an integer matrix as a Python int (multi-word
+    integer Glynn sum, oracle/perm_int.c)."""
+    a = np.ascontiguousarray(np.asarray(matrix), dtype=np.int64)
+    if a.ndim != 2:
+        raise ValueError("matrix must be 2-D")
+    r = np.ascontiguousarray(np.asarray(rows).reshape(-1), dtype=np.int32)
+    c = np.ascontiguousarray(np.asarray(cols).reshape(-1), dtype=np.int32)
+    limbs = np.zeros(8, dtype=np.uint64)
+    _check(
+        lib().pqo_perm_int_exact(
+            _ptr(a, ctypes.c_int64), a.shape[0], a.shape[1], _ptr(r, ctypes.c_int),
+            _ptr(c, ctypes.c_int), n_jobs, _ptr(limbs, ctypes.c_uint64),
+        )
+    )
+    v = 0
+    for k in range(7, -1, -1):
+        v = (v << 64) | int(limbs[k])
+    return v - (1 << 512) if v >> 511 else v
 
 
 def grad_dd(matrix, rows, cols, n_jobs: int = 0) -> np.ndarray:

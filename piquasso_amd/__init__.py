@@ -6,6 +6,7 @@ Public entry points
 -------------------
 permanent(matrix, rows, cols)        drop-in for piquasso._math.permanent.permanent
 permanent_ex(matrix, rows, cols, mode) one permanent with the double-double parts
+permanent_exact(matrix, rows, cols)  the exact integer permanent of an integer matrix
 PermanentBatch / permanent_batch     many uneven permanents in one persistent launch
 permanent_laplace(matrix, rows, cols) drop-in for piquasso._math.permanent.permanent_laplace
 permanent_grad / permanent_grad_batch d perm / d A (grad_perm, any shape)
@@ -21,7 +22,8 @@ generate_lossy_and_partially_distinguishable_samples  sampling from those probab
 marginal_fock_probabilities / generate_marginal_samples  marginals from permanent-polynomial moments
 """
 
-from ._math.permanent import permanent, permanent_laplace, permanent_laplace_batch
+from ._math.permanent import (permanent, permanent_exact, permanent_laplace,
+                              permanent_laplace_batch)
 from .engine import PermanentBatch, permanent_batch, permanent_ex
 from .grad import (PermanentBatchFunction, PermanentFunction, permanent_grad, permanent_grad_batch,
                    torch_permanent, torch_permanent_batch)
@@ -35,7 +37,7 @@ from .sampling import (fock_space_basis, generate_lossy_and_partially_distinguis
 from .marginal import generate_marginal_samples, marginal_fock_probabilities
 
 __all__ = [
-    "permanent", "permanent_laplace", "permanent_ex", "PermanentBatch", "permanent_batch",
+    "permanent", "permanent_exact", "permanent_laplace", "permanent_ex", "PermanentBatch", "permanent_batch",
     "permanent_grad", "permanent_grad_batch", "PermanentFunction", "torch_permanent",
     "output_amplitudes", "output_probabilities", "permanent_laplace_batch",
     "generate_samples", "generate_lossy_samples", "tensor_permanent", "tensor_permanent_batch",

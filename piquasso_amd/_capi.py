@@ -62,6 +62,11 @@ SIGNATURES = {
         _c.c_int,
         [_fp, _c.c_int64, _c.c_int64, _c.c_int64, _i32p, _i32p, _c.c_uint32, _fp, _dp],
     ),
+    "pqp_permanent_int64": (
+        _c.c_int,
+        [_i64p, _c.c_int64, _c.c_int64, _c.c_int64, _i32p, _i32p, _dp,
+         _c.POINTER(_c.c_uint64), _c.c_int32],
+    ),
     "pqp_batch_create": (
         _c.c_int,
         [

@@ -28,6 +28,12 @@ Deliberate differences, where the reference has undefined behaviour: a
 matrix, or a negative multiplicity raise ``ValueError`` instead of reading
 out of bounds.
 
+Integer matrices: a matrix whose selected entries are integers with zero
+imaginary parts and whose Glynn terms may leave the floating-point sweep's
+exact integers is evaluated by the exact multi-modular kernel instead, so
+the result is the exact permanent rounded once (perfect-matching counts at
+any size the engine takes).  ``permanent_exact`` returns the integer itself.
+
 ``permanent_laplace(matrix, rows, cols)`` mirrors the second pybind entry
 (piquasso/_math/permanent.cpp:43-58,79-84; src/permanent_laplace.cpp:43-240):
 element l of the result is perm(A; rows, cols - e_l), for a column with
@@ -49,7 +55,8 @@ import numpy as np
 
 from .. import _capi
 
-__all__ = ["permanent", "permanent_laplace", "permanent_laplace_batch", "resolve_dtype"]
+__all__ = ["permanent", "permanent_exact", "permanent_laplace", "permanent_laplace_batch",
+           "resolve_dtype"]
 
 
 def resolve_dtype(matrix):
@@ -147,6 +154,53 @@ def permanent(matrix, rows, cols):
     )
     _capi.check(st)
     return np.array(complex(out[0], out[1]), dtype=np.complex64)
+
+
+def permanent_exact(matrix, rows, cols) -> int:
+    """The exact permanent of an integer matrix with multiplicities, as a
+    Python ``int`` (``pqp_permanent_int64``: the Glynn sum of
+    src/permanent.cpp:49-264 modulo up to eight primes on the GPU, the
+    integer recovered by the Chinese remainder theorem).  ``matrix`` may be
+    of any integer or bool dtype, or float / complex with integral values and
+    zero imaginary parts.  ``ValueError`` when an entry is not such an
+    integer, or outside the exact path's range (selected entries beyond
+    2^47, column bounds sum_i r_i |a_ij| beyond 2^48, binomial weights
+    prod_i C(r_i, r_i/2) beyond 2^52, a permanent that may exceed 2^390)."""
+    a = np.asarray(matrix)
+    if a.ndim != 2:
+        raise ValueError(f"permanent_exact(): matrix must be 2-D, got shape {a.shape}")
+    if a.dtype.kind in "biu":
+        if a.dtype.kind == "u" and a.size and int(a.max()) > 2**63 - 1:
+            raise ValueError("permanent_exact(): entries beyond 2^47 in magnitude")
+        ai = np.ascontiguousarray(a.astype(np.int64))
+    elif a.dtype.kind in "fc":
+        re = np.real(a)
+        if a.dtype.kind == "c" and np.any(np.imag(a) != 0):
+            raise ValueError("permanent_exact(): entries with non-zero imaginary parts")
+        if not np.all(np.isfinite(re)) or np.any(re != np.rint(re)):
+            raise ValueError("permanent_exact(): entries that are not integers")
+        if re.size and float(np.max(np.abs(re))) > 2.0**62:
+            raise ValueError("permanent_exact(): entries beyond 2^47 in magnitude")
+        ai = np.ascontiguousarray(re.astype(np.int64))
+    else:
+        raise TypeError(f"permanent_exact(): unsupported matrix dtype {a.dtype}")
+    m_r, m_c = ai.shape
+    r = _multiplicities(rows, "rows", m_r)
+    c = _multiplicities(cols, "cols", m_c)
+    out = np.zeros(1, dtype=np.float64)
+    limbs = np.zeros(8, dtype=np.uint64)
+    st = _capi.lib().pqp_permanent_int64(
+        ai.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), m_r, m_c, m_c,
+        r.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+        c.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+        out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+        limbs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), 8,
+    )
+    _capi.check(st)
+    v = 0
+    for k in range(7, -1, -1):
+        v = (v << 64) | int(limbs[k])
+    return v - (1 << 512) if v >> 511 else v
 
 
 def permanent_laplace(matrix, rows, cols):

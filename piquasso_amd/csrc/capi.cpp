@@ -440,6 +440,8 @@ struct SingleState {
   std::mutex mu;
   Exec x;       // column minors, gradients, tensor permanents
   bool init = false;
+  Exec xi;      // exact integer permanents
+  bool init_i = false;
   Exec xs;      // single permanents: results in mapped host memory
   bool init_s = false;
 };
@@ -447,6 +449,290 @@ struct SingleState {
 SingleState &single_state() {
   static SingleState *s = new SingleState();  // never destroyed: outlives atexit HIP teardown
   return *s;
+}
+
+// ---------------------------------------------------- exact integers --
+
+// The eight largest primes below 2^49 (moduli of perm_int.hip).
+constexpr uint64_t kIntPrimes[kIntMaxPrimes] = {
+    562949953421231ull, 562949953421201ull, 562949953421189ull, 562949953421173ull,
+    562949953421131ull, 562949953421111ull, 562949953421099ull, 562949953421047ull};
+
+using u128 = unsigned __int128;
+
+uint64_t pow_mod(uint64_t b, uint64_t e, uint64_t m) {
+  u128 r = 1, x = b % m;
+  while (e) {
+    if (e & 1) r = r * x % m;
+    x = x * x % m;
+    e >>= 1;
+  }
+  return static_cast<uint64_t>(r);
+}
+
+// x (kIntLimbs little-endian 64-bit words) = x * m + add
+constexpr int kIntLimbs = 8;
+void limbs_mul_add(uint64_t *x, uint64_t m, uint64_t add) {
+  u128 carry = add;
+  for (int k = 0; k < kIntLimbs; ++k) {
+    const u128 t = static_cast<u128>(x[k]) * m + carry;
+    x[k] = static_cast<uint64_t>(t);
+    carry = t >> 64;
+  }
+}
+
+// Garner's mixed-radix reconstruction of the residues r[k] (in [0, p_k)),
+// then the symmetric range (-M/2, M/2], as two's complement limbs.
+void crt_limbs(const double *res, int K, uint64_t *x) {
+  uint64_t v[kIntMaxPrimes];
+  for (int k = 0; k < K; ++k) {
+    const uint64_t p = kIntPrimes[k];
+    // value of v_0 + v_1 p_0 + ... + v_{k-1} p_0..p_{k-2} modulo p, and p_0..p_{k-1} mod p
+    u128 acc = 0, prod = 1;
+    for (int i = 0; i < k; ++i) {
+      acc = (acc + static_cast<u128>(v[i]) * prod) % p;
+      prod = prod * (kIntPrimes[i] % p) % p;
+    }
+    const uint64_t r = static_cast<uint64_t>(res[k]);
+    const uint64_t diff = static_cast<uint64_t>((static_cast<u128>(r) + p - acc) % p);
+    v[k] = static_cast<uint64_t>(static_cast<u128>(diff) * pow_mod(static_cast<uint64_t>(prod), p - 2, p) % p);
+  }
+  std::memset(x, 0, kIntLimbs * sizeof(uint64_t));
+  for (int k = K - 1; k >= 0; --k) limbs_mul_add(x, k == K - 1 ? 0 : kIntPrimes[k], v[k]);
+  // M = prod p_k; values at or above M / 2 are negative: x - M
+  uint64_t M[kIntLimbs] = {1};
+  for (int k = 0; k < K; ++k) limbs_mul_add(M, kIntPrimes[k], 0);
+  uint64_t half[kIntLimbs];
+  for (int k = 0; k < kIntLimbs; ++k)
+    half[k] = (M[k] >> 1) | (k + 1 < kIntLimbs ? M[k + 1] << 63 : 0);
+  bool ge = true;  // x >= half ?
+  for (int k = kIntLimbs - 1; k >= 0; --k)
+    if (x[k] != half[k]) {
+      ge = x[k] > half[k];
+      break;
+    }
+  if (ge) {
+    uint64_t borrow = 0;
+    for (int k = 0; k < kIntLimbs; ++k) {
+      const uint64_t a = x[k], b = M[k];
+      x[k] = a - b - borrow;
+      borrow = (a < b) || (a - b < borrow);
+    }
+  }
+}
+
+// x (two's complement limbs) rounded to `bits` <= 53 significant bits, ties
+// to even, as a double.
+double limbs_round(const uint64_t *x, int bits) {
+  uint64_t m[kIntLimbs];
+  const bool neg = (x[kIntLimbs - 1] >> 63) != 0;
+  if (neg) {
+    uint64_t carry = 1;
+    for (int k = 0; k < kIntLimbs; ++k) {
+      m[k] = ~x[k] + carry;
+      carry = (carry && m[k] == 0) ? 1 : 0;
+    }
+  } else {
+    std::memcpy(m, x, sizeof(m));
+  }
+  int top = -1;
+  for (int k = kIntLimbs - 1; k >= 0 && top < 0; --k)
+    if (m[k]) top = 64 * k + 63 - __builtin_clzll(m[k]);
+  if (top < 0) return 0.0;
+  const int lo = top - bits + 1;  // lowest kept bit
+  double v;
+  if (lo <= 0) {
+    v = static_cast<double>(m[0]);  // fewer than `bits` bits: exact
+  } else {
+    auto bit = [&](int pos) -> uint64_t { return (m[pos / 64] >> (pos % 64)) & 1u; };
+    uint64_t keep = 0;
+    for (int pos = top; pos >= lo; --pos) keep = (keep << 1) | bit(pos);
+    const bool half = bit(lo - 1) != 0;
+    bool sticky = false;  // any bit below lo - 1
+    for (int k = 0; k * 64 < lo - 1 && !sticky; ++k) {
+      const int nb = std::min(lo - 1, 64 * (k + 1)) - 64 * k;
+      const uint64_t mask = nb >= 64 ? ~0ull : ((1ull << nb) - 1);
+      sticky = (m[k] & mask) != 0;
+    }
+    if (half && (sticky || (keep & 1))) ++keep;
+    v = std::ldexp(static_cast<double>(keep), lo);
+  }
+  return neg ? -v : v;
+}
+
+// x -= d for an integer-valued double d (two's complement limbs).
+void limbs_sub_double(uint64_t *x, double d) {
+  if (d == 0.0) return;
+  int e = 0;
+  const double f = std::frexp(std::fabs(d), &e);  // |d| = f 2^e, f in [0.5, 1)
+  const uint64_t mant = static_cast<uint64_t>(std::ldexp(f, 53));
+  const int shift = e - 53;  // |d| = mant 2^shift, shift >= 0 or mant has trailing zeros
+  uint64_t y[kIntLimbs] = {};
+  if (shift >= 0) {
+    y[shift / 64] = mant << (shift % 64);
+    if (shift % 64 && shift / 64 + 1 < kIntLimbs) y[shift / 64 + 1] = mant >> (64 - shift % 64);
+  } else {
+    y[0] = mant >> (-shift);
+  }
+  if (d < 0.0) {  // x += |d|
+    u128 carry = 0;
+    for (int k = 0; k < kIntLimbs; ++k) {
+      const u128 s = static_cast<u128>(x[k]) + y[k] + carry;
+      x[k] = static_cast<uint64_t>(s);
+      carry = s >> 64;
+    }
+  } else {
+    uint64_t borrow = 0;
+    for (int k = 0; k < kIntLimbs; ++k) {
+      const uint64_t a = x[k], b = y[k];
+      x[k] = a - b - borrow;
+      borrow = (a < b) || (a - b < borrow);
+    }
+  }
+}
+
+// What the exact path needs to know about a matrix: whether every selected
+// entry is an integer of magnitude <= 2^47 with zero imaginary part, the
+// largest column bound R_j = sum_i r_i |a_ij| (every Glynn row sum
+// |cs_j| <= R_j), log2 of a bound on every Glynn term (the binomial weights
+// prod_i C(r_i, r_i / 2) times prod_j R_j^{c_j}) and log2 of the bound
+// prod_i (sum_j c_j |a_ij|)^{r_i} >= |perm| (the permanent of the expanded
+// matrix is at most the product of its absolute row sums).
+struct IntScan {
+  bool integral = false;
+  double rmax = 0.0, log2_weight = 0.0, log2_term = 0.0, log2_perm = 0.0;
+};
+
+IntScan int_scan(const PermInput &in) {
+  IntScan sc;
+  auto entry = [&](int64_t i, int64_t j, double &re, double &im) {
+    const int64_t o = 2 * (i * in.ld + j);
+    if (in.a_is_float) {
+      re = static_cast<const float *>(in.a)[o];
+      im = static_cast<const float *>(in.a)[o + 1];
+    } else {
+      re = static_cast<const double *>(in.a)[o];
+      im = static_cast<const double *>(in.a)[o + 1];
+    }
+  };
+  const double lim = std::ldexp(1.0, 47);
+  for (int64_t i = 0; i < in.m_r; ++i) {  // usually ends at the first entry
+    if (in.rows[i] <= 0) continue;
+    for (int64_t j = 0; j < in.m_c; ++j) {
+      if (in.cols[j] <= 0) continue;
+      double re, im;
+      entry(i, j, re, im);
+      if (im != 0.0 || !(std::fabs(re) <= lim) || re != std::nearbyint(re)) return sc;
+    }
+  }
+  std::vector<double> rsum(static_cast<size_t>(std::max<int64_t>(in.m_r, 1)), 0.0);
+  double log2_cols = 0.0;
+  for (int64_t j = 0; j < in.m_c; ++j) {
+    if (in.cols[j] <= 0) continue;
+    double rj = 0.0;
+    for (int64_t i = 0; i < in.m_r; ++i) {
+      if (in.rows[i] <= 0) continue;
+      double re, im;
+      entry(i, j, re, im);
+      rj += in.rows[i] * std::fabs(re);
+      rsum[static_cast<size_t>(i)] += in.cols[j] * std::fabs(re);
+    }
+    sc.rmax = std::max(sc.rmax, rj);
+    log2_cols += in.cols[j] * (rj > 0.0 ? std::log2(rj) : -1e9);
+  }
+  for (int64_t i = 0; i < in.m_r; ++i) {
+    if (in.rows[i] <= 0) continue;
+    const double s = rsum[static_cast<size_t>(i)];
+    sc.log2_perm += in.rows[i] * (s > 0.0 ? std::log2(s) : -1e9);
+    const int r = in.rows[i];
+    sc.log2_weight += (std::lgamma(r + 1.0) - std::lgamma(r / 2 + 1.0) - std::lgamma(r - r / 2 + 1.0)) /
+                      std::log(2.0);
+  }
+  sc.log2_term = sc.log2_weight + log2_cols;
+  sc.integral = true;
+  return sc;
+}
+
+// The exact permanent of an integral matrix (int_scan(in).integral) as
+// two's complement limbs: the multi-modular Glynn sweep of perm_int.hip and
+// Garner's reconstruction.  PQP_ERR_RANGE when the row sums or binomial
+// weights leave FP64's exact integers or the permanent may exceed the
+// product of the moduli / 4.
+int run_int(const PermInput &in, const IntScan &sc, uint64_t *v) {
+  // the matrix as contiguous complex128: the plan's B rows
+  std::vector<double> a(static_cast<size_t>(std::max<int64_t>(in.m_r * in.m_c, 1)) * 2, 0.0);
+  for (int64_t i = 0; i < in.m_r; ++i)
+    for (int64_t j = 0; j < in.m_c; ++j)
+      a[static_cast<size_t>(2 * (i * in.m_c + j))] =
+          in.a_is_float ? static_cast<const float *>(in.a)[2 * (i * in.ld + j)]
+                        : static_cast<const double *>(in.a)[2 * (i * in.ld + j)];
+  PlanOptions opt = single_plan_options(0);
+  opt.direct_out = false;
+  opt.expand_cols = 1;  // unit column multiplicities: the product is over n factors
+  opt.pad8 = true;
+  Plan plan;
+  std::string err;
+  const PermInput ind{a.data(), false, in.m_r, in.m_c, in.m_c, in.rows, in.cols};
+  int st = plan_build(std::vector<PermInput>{ind}, opt, plan, err);
+  if (st != PQP_OK) return fail(st, err);
+  if (!sc.integral)
+    return fail(PQP_ERR_RANGE, "matrix entries are not all integers of magnitude <= 2^47");
+  if (sc.rmax > std::ldexp(1.0, 48)) return fail(PQP_ERR_RANGE, "row sums of the matrix exceed 2^48");
+  if (sc.log2_weight > 52.0)
+    return fail(PQP_ERR_RANGE, "binomial weights of the row multiplicities exceed 2^52");
+  // products of G row sums stay below 2^49: exact in FP64 and valid
+  // inputs of the modular multiply
+  const double lr = sc.rmax > 1.0 ? std::log2(sc.rmax) : 0.0;
+  const int group = 4.0 * lr <= 49.0 ? 4 : (2.0 * lr <= 49.0 ? 2 : 1);
+  // moduli: their product (each prime > 2^48.99) must exceed 4 |perm|
+  const int K = std::max(1, static_cast<int>(std::ceil((std::max(sc.log2_perm, 0.0) + 2.0) / 48.99)));
+  if (K > kIntMaxPrimes)
+    return fail(PQP_ERR_RANGE, "the permanent may exceed 2^390 in magnitude: beyond the eight moduli");
+  const ItemDesc &d = plan.items[0];
+  if (d.n == 0) {  // empty selection: 1 (src/permanent.cpp:106-108)
+    std::memset(v, 0, kIntLimbs * sizeof(uint64_t));
+    v[0] = 1;
+    return PQP_OK;
+  }
+  // the moduli table: p, 1/p, 2^-(n-1) mod p
+  const int64_t mod_off = (static_cast<int64_t>(plan.arena.size()) + 63) / 64 * 64;
+  plan.arena.resize(static_cast<size_t>(mod_off) + 3 * kIntMaxPrimes * sizeof(double), 0);
+  double *mt = reinterpret_cast<double *>(plan.arena.data() + mod_off);
+  for (int k = 0; k < kIntMaxPrimes; ++k) {
+    const uint64_t p = kIntPrimes[k];
+    mt[k] = static_cast<double>(p);
+    mt[kIntMaxPrimes + k] = 1.0 / static_cast<double>(p);
+    const uint64_t inv2 = pow_mod((p + 1) / 2, static_cast<uint64_t>(d.n > 0 ? d.n - 1 : 0), p);
+    mt[2 * kIntMaxPrimes + k] = static_cast<double>(inv2);
+  }
+  if ((st = require_device()) != PQP_OK) return st;
+  SingleState &S = single_state();
+  std::lock_guard<std::mutex> lock(S.mu);
+  Exec &x = S.xi;
+  if (!S.init_i) {
+    PQP_HIP(hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking));
+    x.own_stream = true;
+    S.init_i = true;
+  }
+  x.plan = std::move(plan);
+  index_units(x);
+  // 8 residues per unit and per item: twice the forward plan's partials
+  x.plan.n_partials = 2 * std::max<int64_t>(x.plan.n_partials, 1);
+  x.plan.n_out = 2 * std::max<int64_t>(x.plan.n_out, 1);
+  if ((st = upload(x)) != PQP_OK) return st;
+  const LaunchArgs la = x.args();
+  IntArgs ia;
+  ia.mod_off = mod_off;
+  ia.n_primes = K;
+  ia.group = group;
+  for (const Plan::Range &r : x.plan.ranges)
+    PQP_HIP(launch_int(x.plan.items[0].mcp, la, ia, r.first, r.count, x.stream));
+  PQP_HIP(launch_int_finalize(la, ia, 1, x.stream));
+  double res[kIntMaxPrimes];
+  PQP_HIP(hipMemcpyAsync(res, x.results.p, sizeof(res), hipMemcpyDeviceToHost, x.stream));
+  PQP_HIP(hipStreamSynchronize(x.stream));
+  crt_limbs(res, K, v);
+  return PQP_OK;
 }
 
 int run_single(const PermInput &in, uint32_t flags, double *out2, double *out_dd4) {
@@ -457,6 +743,33 @@ int run_single(const PermInput &in, uint32_t flags, double *out2, double *out_dd
   std::string err;
   int st = plan_build(std::vector<PermInput>{in}, opt, plan, err);
   if (st != PQP_OK) return fail(st, err);
+  if (!(flags & PQP_FLAG_PLAIN)) {
+    // Integer matrices whose Glynn terms may leave the exact integers of
+    // the floating-point sweep (FP32 products: 2^24; FP64 products summed
+    // in runs of kDdFold before the double-double fold: 2^53 / kDdFold)
+    // take the multi-modular path: the exact permanent, rounded once.
+    const IntScan sc = int_scan(in);
+    const bool fp32 = in.a_is_float || (flags & PQP_FLAG_FP32);
+    if (sc.integral && sc.log2_term >= (fp32 ? 24.0 : 53.0 - std::log2(double(kDdFold)))) {
+      uint64_t v[kIntLimbs];
+      st = run_int(in, sc, v);
+      if (st == PQP_OK) {
+        out2[0] = limbs_round(v, fp32 ? 24 : 53);
+        out2[1] = 0.0;
+        if (out_dd4) {
+          const double hi = limbs_round(v, 53);
+          limbs_sub_double(v, hi);
+          out_dd4[0] = hi;
+          out_dd4[1] = limbs_round(v, 53);
+          out_dd4[2] = out_dd4[3] = 0.0;
+        }
+        return PQP_OK;
+      }
+      if (st != PQP_ERR_RANGE) return st;
+      // beyond the moduli or the weight bound: the floating-point sweep
+      g_error.clear();
+    }
+  }
   if ((st = require_device()) != PQP_OK) return st;
   SingleState &S = single_state();
   std::lock_guard<std::mutex> lock(S.mu);
@@ -764,6 +1077,7 @@ size_t align256(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
 
 }  // namespace
 
+
 namespace {
 
 int grad_batch_minors(const std::vector<PermInput> &inputs, double *perms, double *const *grads) {
@@ -882,6 +1196,40 @@ int pqp_permanent_c64(const float *matrix, int64_t m_r, int64_t m_c, int64_t ld,
   if (st != PQP_OK) return st;
   out[0] = static_cast<float>(o[0]);
   out[1] = static_cast<float>(o[1]);
+  return PQP_OK;
+}
+
+int pqp_permanent_int64(const int64_t *matrix, int64_t m_r, int64_t m_c, int64_t ld,
+                        const int32_t *rows, const int32_t *cols, double *out, uint64_t *limbs,
+                        int32_t n_limbs) {
+  CallRange range_("pqp_permanent_int64");
+  if (!out) return fail(PQP_ERR_INVALID, "out is null");
+  if (limbs && n_limbs < kIntLimbs)
+    return fail(PQP_ERR_INVALID, "limbs needs room for 8 words (512 bits)");
+  if (m_r < 0 || m_c < 0 || (m_r * m_c > 0 && !matrix) || (m_r > 0 && !rows) ||
+      (m_c > 0 && !cols) || (m_r > 0 && ld < m_c))
+    return fail(PQP_ERR_INVALID, "invalid shape or null pointer");
+  for (int64_t i = 0; i < m_r; ++i)
+    if (rows[i] < 0) return fail(PQP_ERR_INVALID, "negative row multiplicity");
+  for (int64_t j = 0; j < m_c; ++j)
+    if (cols[j] < 0) return fail(PQP_ERR_INVALID, "negative column multiplicity");
+  // the matrix as complex128 with zero imaginary parts; entries beyond
+  // 2^47 (which a double may not hold exactly) are refused by the scan when
+  // they are selected
+  std::vector<double> a(static_cast<size_t>(std::max<int64_t>(m_r * m_c, 1)) * 2, 0.0);
+  for (int64_t i = 0; i < m_r; ++i)
+    for (int64_t j = 0; j < m_c; ++j) {
+      const int64_t e = matrix[i * ld + j];
+      const int64_t lim = int64_t(1) << 47;
+      a[static_cast<size_t>(2 * (i * m_c + j))] =
+          static_cast<double>(e > lim ? lim + 1 : (e < -lim ? -lim - 1 : e));
+    }
+  const PermInput in{a.data(), false, m_r, m_c, m_c, rows, cols};
+  uint64_t v[kIntLimbs];
+  const int st = run_int(in, int_scan(in), v);
+  if (st != PQP_OK) return st;
+  out[0] = limbs_round(v, 53);
+  if (limbs) std::memcpy(limbs, v, sizeof(v));
   return PQP_OK;
 }
 

@@ -75,6 +75,13 @@ hipError_t launch_grad(int bucket, const LaunchArgs &a, int first_unit, int n_un
 hipError_t launch_grad_finalize(const LaunchArgs &a, int n_items, int max_values,
                                 hipStream_t stream);
 
+// Exact integer permanents (perm_int.hip): one launch per padded width
+// (a multiple of 8), then the finalize; 8 residues per unit and per item.
+hipError_t launch_int(int mcp, const LaunchArgs &a, const IntArgs &ia, int first_unit,
+                      int n_units, hipStream_t stream);
+hipError_t launch_int_finalize(const LaunchArgs &a, const IntArgs &ia, int n_items,
+                               hipStream_t stream);
+
 // Tensor permanents (tensor_perm.hip): one launch per matrix order n over
 // units [first_unit, first_unit + n_units), then one finalize for all items.
 hipError_t launch_tensor_p0(int n, const TensorArgs &a, int first_unit, int n_units, hipStream_t s);

@@ -283,7 +283,11 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     }
     const int mc = static_cast<int>(colsrc.size());
     int mcp = 0;
-    const int bucket = bucket_for_width(mc, opt.fp32_products, &mcp);
+    int bucket = bucket_for_width(mc, opt.fp32_products, &mcp);
+    if (opt.pad8 && bucket >= 0) {
+      mcp = static_cast<int>(align_up(mc, 8));
+      bucket = mcp / 8 - 1;  // one range per padded width
+    }
     if (opt.minors && mcp > 32) {
       err = "item " + std::to_string(k) + ": column minors support at most 32 columns";
       return kErrRange;

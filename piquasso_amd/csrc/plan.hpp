@@ -51,6 +51,7 @@ struct PlanOptions {
                                 // uncut product (m_c + 1 values per item)
   int64_t total_units = 0;      // spread about this many waves over the batch, in
                                 // proportion to each permanent's cost (0 = off)
+  bool pad8 = false;            // kernel width padded to a multiple of 8 (exact integer path)
   bool grad = false;            // gradient items: the permanent and d perm / d B for every
                                 // row of B from one sweep (perm_grad.hip); outputs
                                 // 1 + rows_B * m_c values per item

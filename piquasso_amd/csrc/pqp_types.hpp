@@ -38,6 +38,16 @@ enum ItemFlags : int {
 };
 
 constexpr int kGradMaxInner = 16;  // inner digits of a gradient item (per-row LDS accumulators)
+constexpr int kIntMaxPrimes = 8;   // moduli of the exact integer permanent
+
+// Exact integer permanents (perm_int.hip): the arena holds, at mod_off,
+// kIntMaxPrimes primes, their reciprocals and 2^-(n-1) modulo each
+// (doubles); products are formed in exact groups of `group` columns.
+struct IntArgs {
+  int64_t mod_off;
+  int32_t n_primes;
+  int32_t group;
+};
 
 // Doubles per work unit of a gradient item: the permanent's dd partial (4),
 // then one FP64 value per (row of B, real component) with the components of

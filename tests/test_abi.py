@@ -133,6 +133,55 @@ def test_batch_entry_errors_precede_the_device():
         pa.tensor_permanent(np.zeros((2, 3)), 0.5)
 
 
+def test_integer_entry_errors_precede_the_device():
+    """pqp_permanent_int64 checks its arguments and the exact path's range
+    before it needs a device; an input it accepts reaches the device check."""
+    import piquasso_amd as pa
+
+    L = _capi.lib()
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    dp = ctypes.POINTER(ctypes.c_double)
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+
+    def call(a, rows, cols, out=True, n_limbs=8):
+        a = np.ascontiguousarray(a, dtype=np.int64)
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        c = np.ascontiguousarray(cols, dtype=np.int32)
+        o = np.zeros(1)
+        limbs = np.zeros(8, dtype=np.uint64)
+        return L.pqp_permanent_int64(a.ctypes.data_as(i64p), a.shape[0], a.shape[1], a.shape[1],
+                                     r.ctypes.data_as(i32p), c.ctypes.data_as(i32p),
+                                     o.ctypes.data_as(dp) if out else None,
+                                     limbs.ctypes.data_as(u64p), n_limbs)
+
+    one2 = [1, 1]
+    assert call(np.eye(2), one2, one2, out=False) == _capi.PQP_ERR_INVALID
+    assert call(np.eye(2), one2, one2, n_limbs=4) == _capi.PQP_ERR_INVALID
+    assert call(np.eye(2), [1, -1], [0, 0]) == _capi.PQP_ERR_INVALID
+    assert call(np.eye(2), one2, [2, 1]) == _capi.PQP_ERR_SUM_MISMATCH
+    big = np.ones((2, 2), dtype=np.int64)
+    big[0, 0] = 2**48
+    assert call(big, one2, one2) == _capi.PQP_ERR_RANGE          # a selected entry > 2^47
+    assert "2^47" in L.pqp_last_error().decode()
+    assert call(np.full((3, 1), 2**47), [1, 1, 1], [3]) == _capi.PQP_ERR_RANGE  # column bound > 2^48
+    assert call(np.ones((1, 1)), [60], [60]) == _capi.PQP_ERR_RANGE  # C(60, 30) > 2^52
+    assert "binomial" in L.pqp_last_error().decode()
+    # |perm| bound (20 * 2^20)^20 = 2^486
+    assert call(np.full((20, 20), 2**20), np.ones(20), np.ones(20)) == _capi.PQP_ERR_RANGE
+    assert "2^390" in L.pqp_last_error().decode()
+    n_dev = ctypes.c_int(0)
+    if L.pqp_device_info(ctypes.byref(n_dev), None, None, 0) != _capi.PQP_OK:
+        # unselected large entries are never read; the rest is the device's
+        assert call(big, [0, 1], [0, 1]) == _capi.PQP_ERR_DEVICE
+    with pytest.raises(ValueError, match="not integers"):
+        pa.permanent_exact(np.array([[0.5]]), [1], [1])
+    with pytest.raises(ValueError, match="imaginary"):
+        pa.permanent_exact(np.array([[1j]]), [1], [1])
+    with pytest.raises(RuntimeError):
+        pa.permanent_exact(np.eye(2, dtype=int), [1, 1], [2, 1])
+
+
 def test_empty_selection_needs_no_device():
     """perm of an empty selection is 1 (src/permanent.cpp:106-108); the
     Laplace entry returns the reference's [1] for degenerate inputs."""
@@ -140,6 +189,8 @@ def test_empty_selection_needs_no_device():
 
     assert pa.permanent_laplace(np.ones((2, 2)), [0, 0], [0, 1]).tolist() == [1]
     assert pa.permanent_laplace(np.zeros((0, 3)), [], [1, 0, 0]).tolist() == [1]
+    assert pa.permanent_exact(np.zeros((0, 0), dtype=int), [], []) == 1
+    assert pa.permanent_exact(np.ones((2, 2), dtype=int), [0, 0], [0, 0]) == 1
 
 
 def test_no_kernel_spills_to_scratch():

@@ -418,3 +418,128 @@ def test_laplace_dd_exact_minors_and_expansion(oracle):
                 # integer entries: every term and partial sum is exact
                 assert lap[l] == complex(float(er), float(ei)), (t, x, l)
             assert np.sum(c * a[x] * lap) == full, (t, x)
+
+
+# ----------------------------------------------------- exact integers --
+
+
+def _perm_int_bruteforce(a, rows, cols):
+    """Exact permanent of the expanded integer matrix by the definition."""
+    ri = np.repeat(np.arange(len(rows)), np.asarray(rows, int))
+    ci = np.repeat(np.arange(len(cols)), np.asarray(cols, int))
+    b = [[int(a[i][j]) for j in ci] for i in ri]
+    n = len(b)
+    total = 0
+    for p in itertools.permutations(range(n)):
+        prod = 1
+        for i in range(n):
+            prod *= b[i][p[i]]
+            if not prod:
+                break
+        total += prod
+    return total
+
+
+def _perm_int_ryser(b):
+    """Ryser's formula in Python integers (an independent method):
+    perm(B) = (-1)^n sum_S (-1)^|S| prod_i sum_{j in S} b_ij."""
+    n = len(b)
+    total = 0
+    for mask in range(1, 1 << n):
+        cols = [j for j in range(n) if mask >> j & 1]
+        prod = 1
+        for i in range(n):
+            prod *= sum(b[i][j] for j in cols)
+            if not prod:
+                break
+        total += (-1) ** len(cols) * prod
+    return (-1) ** n * total
+
+
+def _derangements(n):
+    d = [1, 0]
+    for k in range(2, n + 1):
+        d.append((k - 1) * (d[k - 1] + d[k - 2]))
+    return d[n]
+
+
+def test_perm_int_exact_matches_bruteforce(oracle):
+    """The integer oracle equals the definition on random integer matrices
+    with multiplicities, including entries of 2^40 whose terms need several
+    64-bit words."""
+    rng = np.random.default_rng(21)
+    for t in range(40):
+        m_r, m_c = (int(x) for x in rng.integers(1, 5, size=2))
+        n = int(rng.integers(1, 7))
+        hi = 2**40 if t % 4 == 0 else 4
+        a = rng.integers(-hi, hi + 1, size=(m_r, m_c), dtype=np.int64)
+        r = np.bincount(rng.integers(0, m_r, n), minlength=m_r)
+        c = np.bincount(rng.integers(0, m_c, n), minlength=m_c)
+        assert oracle.perm_int_exact(a, r, c) == _perm_int_bruteforce(a, r, c), t
+
+
+def test_perm_int_exact_matches_ryser(oracle):
+    rng = np.random.default_rng(22)
+    for n in (8, 10, 12):
+        b = rng.integers(0, 2, size=(n, n), dtype=np.int64)
+        b[rng.random((n, n)) < 0.2] = -1
+        one = np.ones(n, dtype=int)
+        assert oracle.perm_int_exact(b, one, one) == _perm_int_ryser(b.tolist()), n
+
+
+def test_perm_int_exact_closed_forms(oracle):
+    """perm(J_n) = n!, perm(J_n - I) = D_n (derangements), the all-ones
+    matrix with multiplicities is n! whatever the multiplicities, and the
+    reference's integer case (tests/_math/test_permanent.py:252-282) is 0."""
+    for n in (1, 2, 5, 14, 18, 20):
+        one = np.ones(n, dtype=int)
+        assert oracle.perm_int_exact(np.ones((n, n), np.int64), one, one) == math.factorial(n)
+        assert oracle.perm_int_exact(np.ones((n, n), np.int64) - np.eye(n, dtype=np.int64), one,
+                                     one) == _derangements(n)
+    assert oracle.perm_int_exact(np.ones((3, 4), np.int64), [5, 0, 7], [2, 3, 4, 3]) == \
+        math.factorial(12)
+    a = np.array([[0, 0, -1, -1, -1, 0, -1, 0, 0, -1], [-1, -1, -1, -1, 0, 0, -1, -1, -1, -1],
+                  [-1, 0, -1, -1, -1, 0, 0, -1, 0, -1], [-1, 0, -1, -1, -1, -1, 0, -1, -1, -1],
+                  [-1, 0, 0, -1, -1, -1, -1, -1, 0, -1], [0, -1, 0, 0, -1, -1, -1, -1, 0, -1],
+                  [-1, 0, 0, 0, -1, -1, -1, -1, 0, -1], [0, 0, -1, -1, -1, -1, -1, -1, 0, 0],
+                  [0, 0, 0, -1, -1, -1, 0, 0, -1, -1], [0, 0, 0, 0, -1, 0, -1, -1, -1, 0]],
+                 dtype=np.int64)
+    rows = [0, 0, 0, 0, 0, 3, 4, 0, 3, 0]
+    cols = [1, 0, 0, 0, 0, 0, 1, 7, 0, 1]
+    assert oracle.perm_int_exact(a, rows, cols) == 0
+    # n = 20 all-ones: beyond FP64's exact integers, where the double-double
+    # oracle is only close
+    one = np.ones(20, dtype=int)
+    dd = oracle.perm_dd(np.ones((20, 20)), one, one)
+    assert abs(dd.real - math.factorial(20)) <= 1e-15 * math.factorial(20)
+
+
+def test_perm_int_exact_errors(oracle):
+    with pytest.raises(Exception):
+        oracle.perm_int_exact(np.ones((2, 2), np.int64), [1, 1], [2, 1])
+    # a term beyond 511 bits is refused, not wrapped
+    with pytest.raises(Exception):
+        oracle.perm_int_exact(np.full((1, 1), 2**40, np.int64), [13], [13])
+
+
+def test_perm_int_planted_mistakes_are_caught(oracle):
+    """Plausible slips in an integer Glynn sum (a dropped sign, a missing
+    final shift by n - 1) fail the closed form perm(J_n) = n!."""
+    def glynn(b, mistake=None):
+        n = len(b)
+        total = 0
+        for mask in range(1 << (n - 1)):
+            d = [1] + [(-1) ** (mask >> k & 1) for k in range(n - 1)]
+            prod = 1
+            for j in range(n):
+                prod *= sum(d[i] * b[i][j] for i in range(n))
+            sign = 1 if mistake == "sign" else (-1) ** bin(mask).count("1")
+            total += sign * prod
+        return total if mistake == "shift" else total >> (n - 1)
+
+    n = 6
+    j = np.ones((n, n), dtype=int).tolist()
+    assert glynn(j) == math.factorial(n)
+    assert oracle.perm_int_exact(np.array(j, np.int64), [1] * n, [1] * n) == glynn(j)
+    for m in ("sign", "shift"):
+        assert glynn(j, m) != math.factorial(n), m
