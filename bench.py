@@ -25,6 +25,8 @@ Printed as ONE JSON line (rank 0):
                   photon round), checked shot for shot against the reference's
                   own permanent_laplace driven sequentially on the host
   rel_err_max     vs the double-double oracle values committed with the inputs
+  error_bound     the per-result error bounds the sweep kernels report, relative to
+                  |perm|, and whether every result lies within its bound of the oracle
   roofline        the sweep kernels: algorithmic FLOP / kernel time vs 78.6 TFLOP/s;
                   traffic = HBM bytes per evaluation from the committed rocprofv3
                   FETCH_SIZE + WRITE_SIZE summary in profiles/
@@ -636,6 +638,17 @@ def main():
         batch.sync()
         walls[mode] = time.perf_counter() - t0
         rels[mode] = np.abs(batch.results() - ref) / np.abs(ref)
+    # per-result error bounds of the last evaluation (term magnitudes summed
+    # by the static sweep kernels) against the oracle's distance
+    bounds = batch.error_bounds()
+    abs_err = np.abs(batch.results() - ref)
+    error_bound = {
+        "rel_max": float(np.max(bounds / np.abs(ref))),
+        "rel_median": float(np.median(bounds / np.abs(ref))),
+        "all_hold": bool(np.all(abs_err <= bounds)),
+        "what": "4 (n+2) u sum|terms| / 2^(n-1) + final rounding per result, "
+                "from the sweep kernels' term magnitudes; all_hold: |result - oracle| <= bound",
+    }
     launch = min(walls, key=walls.get)
     wall = walls[launch]
     rel = np.maximum(rels["graph"], rels["eager"])
@@ -662,11 +675,13 @@ def main():
     t32 = b32.time("kernel", reps=5)
     f32 = W.glynn_flops(r32, c32)
     t32_ms = float(np.median(t32))
+    _, _, bound32 = pa.permanent_ex(a32, r32, c32, with_bound=True)
     n32 = {
         "ms": t32_ms,
         "tflops": f32 / (t32_ms * 1e-3) / 1e12,
         "frac_fp64_peak": f32 / (t32_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
         "rel_err": abs(v32 - ref32) / abs(ref32),
+        "rel_error_bound": bound32 / abs(ref32),
         "flops": f32,
     }
 
@@ -675,6 +690,7 @@ def main():
     b32b.run("graph")
     b32b.sync()
     v32b = b32b.results()
+    bounds32b = b32b.error_bounds()
     g32 = b32b.time("graph", reps=max(3, min(args.steps, 10)))
     k32 = b32b.time("kernel", reps=max(3, min(args.steps, 10)))
     b32b.close()
@@ -685,6 +701,8 @@ def main():
         "kernel_ms": float(np.median(k32)),
         "rel_err_max": float(rel32.max()),
         "rel_err_median": float(np.median(rel32)),
+        "error_bound_rel_max": float(np.max(bounds32b / np.abs(ref))),
+        "error_bounds_hold": bool(np.all(np.abs(v32b - ref) <= bounds32b)),
         "tflops": flops_batch / (float(np.median(k32)) * 1e-3) / 1e12,
         "frac_fp32_peak": flops_batch / (float(np.median(k32)) * 1e-3) / 1e12 / 157.3,
         "what": "the batched workload with FP32 row sums and products (FP64 + double-double "
@@ -759,6 +777,7 @@ def main():
             "device": f"{arch} ({cus} CUs)",
         },
         "rel_err_max": float(rel.max()),
+        "error_bound": error_bound,
         "launch": launch,
         "permanents_per_s_by_launch": {m: len(mats) * args.steps / w for m, w in walls.items()},
         "graph_ms_median": float(np.median(graph_ms)),

@@ -63,6 +63,21 @@ const char *pqp_version(void);
 /* Message of the last failure on the calling thread ("" if none). */
 const char *pqp_last_error(void);
 
+/* Error bound of the last successful single-permanent call
+ * (pqp_permanent_c128, _c64, _c128_noalloc) on the calling thread.  The
+ * floating-point sweeps sum the magnitudes of their terms beside the terms;
+ * the bound is 4 (n + 2) u sum_t |term_t| / 2^(n-1), u the unit roundoff of
+ * the row sums and products (2^-53, or 2^-24 with FP32 products), plus the
+ * final rounding of the result: the forward-error bound of a sum of
+ * products whose n factors each carry a relative error of order u.  A row
+ * sum that itself cancels can carry more, so this is an a-posteriori
+ * estimate, not a proof; the GPU tests check it against the double-double
+ * oracle.  For a result of the exact integer path it is the final rounding
+ * alone.  NaN after a PQP_FLAG_PLAIN call (the plain kernel keeps no
+ * magnitudes).  No counterpart in the reference, whose permanent_cpp
+ * returns the value alone. */
+double pqp_last_error_bound(void);
+
 /* Device 0 properties: returns PQP_OK and fills the outputs, or
  * PQP_ERR_DEVICE when no HIP device is usable.  `name` may be null. */
 int pqp_device_info(int *n_devices, int *cu_count, char *name, int name_len);
@@ -190,6 +205,13 @@ int pqp_batch_sync(pqp_batch *b);
 /* Copy the results of the last run: out = 2 doubles per item, out_dd = 4
  * doubles per item (may be null).  Synchronises first. */
 int pqp_batch_results(pqp_batch *b, double *out, double *out_dd);
+
+/* Error bounds of the last evaluation, one per item: pqp_last_error_bound's
+ * estimate of |result k - perm k| from the term magnitudes the forward
+ * sweeps sum; NaN for every item when the last evaluation kept none (mode
+ * PQP_RUN_PLAIN without FP32 products, gradient batches of at most 32
+ * particles). */
+int pqp_batch_error_bounds(pqp_batch *b, double *bounds);
 
 /* Time `reps` evaluations with HIP events recorded on the batch stream:
  * ms[k] = elapsed milliseconds of evaluation k. */

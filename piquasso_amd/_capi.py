@@ -48,6 +48,7 @@ _vp = _c.c_void_p
 SIGNATURES = {
     "pqp_version": (_c.c_char_p, []),
     "pqp_last_error": (_c.c_char_p, []),
+    "pqp_last_error_bound": (_c.c_double, []),
     "pqp_device_info": (_c.c_int, [_c.POINTER(_c.c_int), _c.POINTER(_c.c_int), _c.c_char_p, _c.c_int]),
     "pqp_debug_fail_allocation": (_c.c_int, [_c.c_int64]),
     "pqp_permanent_c128": (
@@ -78,6 +79,7 @@ SIGNATURES = {
     "pqp_batch_run": (_c.c_int, [_vp, _c.c_int]),
     "pqp_batch_sync": (_c.c_int, [_vp]),
     "pqp_batch_results": (_c.c_int, [_vp, _dp, _dp]),
+    "pqp_batch_error_bounds": (_c.c_int, [_vp, _dp]),
     "pqp_batch_time": (_c.c_int, [_vp, _c.c_int, _c.c_int, _fp]),
     "pqp_batch_stats": (_c.c_int, [_vp, _dp, _i64p, _i64p, _i64p, _i64p]),
     "pqp_batch_destroy": (None, [_vp]),

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <limits>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -31,6 +32,7 @@ namespace {
 using namespace pqp;
 
 thread_local std::string g_error;
+thread_local double g_last_bound = std::numeric_limits<double>::quiet_NaN();
 
 int fail(int code, const std::string &msg) {
   g_error = msg;
@@ -236,12 +238,13 @@ struct Exec {
   Plan plan;
   std::vector<int32_t> item_first_unit;  // first unit of each item in plan.units
   DevBuf arena, partials, heads;
-  DevBuf results;   // 2 doubles per result, then 4 (dd parts) per result
+  DevBuf results;   // 2 doubles per result, 4 (dd parts) per result, 1 (term magnitudes) per result
   bool mapped_results = false;  // results in results_m (host memory) instead
   MappedBuf results_m;
   HostBuf stage;    // pinned staging for the arena upload and result downloads
   size_t n_results = 0;
   bool heads_clean = false;  // queue heads zeroed since (re)allocation
+  bool abs_valid = false;    // the last evaluation left term magnitudes
   hipStream_t stream = nullptr;
   bool own_stream = false;
   uint32_t flags = 0;
@@ -259,6 +262,8 @@ struct Exec {
     double *r = static_cast<double *>(mapped_results ? results_m.dev : results.p);
     a.out = r;
     a.out_dd = r + 2 * n_results;
+    a.out_abs = r + 6 * n_results;
+    a.part_abs = static_cast<double *>(partials.p) + 4 * std::max<int64_t>(plan.n_partials, 1);
     a.queue_head = static_cast<int *>(heads.p);
     a.n_units = static_cast<int32_t>(plan.units.size());
     a.n_items = static_cast<int32_t>(plan.items.size());
@@ -280,10 +285,11 @@ int upload(Exec &x) {
   const size_t n_out = static_cast<size_t>(std::max<int64_t>(p.n_out, 1));
   int st;
   if ((st = x.arena.ensure(p.arena.size())) != PQP_OK) return st;
-  if ((st = x.partials.ensure(std::max<int64_t>(p.n_partials, 1) * 4 * sizeof(double))) != PQP_OK) return st;
+  // partials: 4 doubles per slot, then 1 (term magnitudes) per slot
+  if ((st = x.partials.ensure(std::max<int64_t>(p.n_partials, 1) * 5 * sizeof(double))) != PQP_OK) return st;
   if (x.mapped_results) {
-    if ((st = x.results_m.ensure(n_out * 6 * sizeof(double))) != PQP_OK) return st;
-  } else if ((st = x.results.ensure(n_out * 6 * sizeof(double))) != PQP_OK) {
+    if ((st = x.results_m.ensure(n_out * 7 * sizeof(double))) != PQP_OK) return st;
+  } else if ((st = x.results.ensure(n_out * 7 * sizeof(double))) != PQP_OK) {
     return st;
   }
   x.n_results = n_out;
@@ -325,6 +331,11 @@ int enqueue(Exec &x, int mode, hipEvent_t ev_begin, hipEvent_t ev_end) {
   if (ev_begin) PQP_HIP(hipEventRecord(ev_begin, x.stream));
   if ((p.minors || p.grad) && mode == PQP_RUN_PLAIN)
     return fail(PQP_ERR_INVALID, "no plain mode for column minors or gradients");
+  // compensated static forward launches also leave the term magnitudes
+  x.abs_valid = !p.minors && !p.grad &&
+                (mode == PQP_RUN_PLAIN ? p.fp32
+                                       : ((x.flags & PQP_FLAG_STATIC) &&
+                                          (!(x.flags & PQP_FLAG_PLAIN) || p.fp32)));
   if (mode == PQP_RUN_PLAIN) {
     for (int k = 0; k < n_items; ++k) {
       const ItemDesc &d = p.items[k];
@@ -756,6 +767,7 @@ int run_single(const PermInput &in, uint32_t flags, double *out2, double *out_dd
       if (st == PQP_OK) {
         out2[0] = limbs_round(v, fp32 ? 24 : 53);
         out2[1] = 0.0;
+        g_last_bound = std::fabs(out2[0]) * (fp32 ? 0x1p-24 : 0x1p-53);  // one rounding
         if (out_dd4) {
           const double hi = limbs_round(v, 53);
           limbs_sub_double(v, hi);
@@ -780,6 +792,8 @@ int run_single(const PermInput &in, uint32_t flags, double *out2, double *out_dd
     x.mapped_results = true;
     S.init_s = true;
   }
+  const int n = plan.items[0].n;
+  const bool fp32 = plan.fp32;
   x.plan = std::move(plan);
   index_units(x);
   x.flags = (flags & PQP_FLAG_PLAIN) | PQP_FLAG_STATIC;
@@ -788,13 +802,17 @@ int run_single(const PermInput &in, uint32_t flags, double *out2, double *out_dd
                     nullptr)) != PQP_OK)
     return st;
   PQP_HIP(hipStreamSynchronize(x.stream));
-  // one item: its 2 result doubles and 4 dd parts are contiguous
-  // (results = [out (2 per result)][out_dd (4 per result)], one result),
+  // one item: results = [out (2)][out_dd (4)][term magnitudes (1)],
   // written by the kernels into mapped host memory
   const double *res = static_cast<const double *>(x.results_m.host);
   out2[0] = res[0];
   out2[1] = res[1];
   if (out_dd4) std::memcpy(out_dd4, res + 2, 4 * sizeof(double));
+  // (the plain baseline runs the non-compensated kernel, which keeps no
+  // term magnitudes)
+  g_last_bound = n == 0 ? 0.0
+                 : x.abs_valid ? sweep_error_bound(n, fp32, res[6], res[0], res[1])
+                               : std::numeric_limits<double>::quiet_NaN();
   return PQP_OK;
 }
 
@@ -1136,6 +1154,13 @@ namespace pqp {
 
 int capi_fail(int code, const std::string &msg) { return fail(code, msg); }
 
+double sweep_error_bound(int n, bool fp32, double m, double re, double im) {
+  const double u = fp32 ? 0x1p-24 : 0x1p-53;
+  return 4.0 * (std::max(n, 0) + 2) * u * m + 0x1p-53 * std::hypot(re, im);
+}
+
+void capi_set_last_bound(double bound) { g_last_bound = bound; }
+
 int capi_require_device() { return require_device(); }
 
 PlanOptions single_plan_options(uint32_t flags) {
@@ -1157,6 +1182,8 @@ extern "C" {
 const char *pqp_version(void) { return "pqp-permanent 0.1.0 (gfx950)"; }
 
 const char *pqp_last_error(void) { return g_error.c_str(); }
+
+double pqp_last_error_bound(void) { return g_last_bound; }
 
 int pqp_device_info(int *n_devices, int *cu_count, char *name, int name_len) {
   DeviceState &d = device_state();
@@ -1196,6 +1223,7 @@ int pqp_permanent_c64(const float *matrix, int64_t m_r, int64_t m_c, int64_t ld,
   if (st != PQP_OK) return st;
   out[0] = static_cast<float>(o[0]);
   out[1] = static_cast<float>(o[1]);
+  g_last_bound += 0x1p-24 * std::hypot(static_cast<double>(out[0]), static_cast<double>(out[1]));
   return PQP_OK;
 }
 
@@ -1725,6 +1753,30 @@ int pqp_batch_results(pqp_batch *b, double *out, double *out_dd) {
   if (out_dd && n)
     PQP_HIP(hipMemcpy(out_dd, static_cast<double *>(b->x.results.p) + 2 * b->x.n_results,
                       n * 4 * sizeof(double), hipMemcpyDeviceToHost));
+  return PQP_OK;
+}
+
+int pqp_batch_error_bounds(pqp_batch *b, double *bounds) {
+  CallRange range_("pqp_batch_error_bounds");
+  if (!b || (!bounds && !b->x.plan.items.empty())) return fail(PQP_ERR_INVALID, "invalid arguments");
+  if (!b->ran) return fail(PQP_ERR_INVALID, "batch has not been run");
+  const size_t n = b->x.plan.items.size();
+  if (!n) return PQP_OK;
+  PQP_HIP(hipStreamSynchronize(b->x.stream));
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  // the gradient kernels and the plain baseline keep no term magnitudes
+  if (!b->x.abs_valid) {
+    for (size_t k = 0; k < n; ++k) bounds[k] = nan;
+    return PQP_OK;
+  }
+  const size_t nr = b->x.n_results;
+  std::vector<double> all(7 * nr);
+  PQP_HIP(hipMemcpy(all.data(), b->x.results.p, all.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t k = 0; k < n; ++k) {
+    const int64_t o = b->x.plan.info[k].out_offset;
+    bounds[k] = sweep_error_bound(b->x.plan.items[k].n, b->x.plan.fp32, all[6 * nr + o],
+                                  all[2 * o], all[2 * o + 1]);
+  }
   return PQP_OK;
 }
 

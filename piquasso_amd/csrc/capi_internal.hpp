@@ -22,4 +22,14 @@ int capi_require_device();
 // write their result from the sweep kernel.
 PlanOptions single_plan_options(uint32_t flags);
 
+// Bound on |computed - perm| of a floating-point sweep from its term
+// magnitudes m = sum_t |term_t| / 2^(n-1) (as the kernels accumulate them):
+// 4 (n + 2) u m, u the unit roundoff of the row sums and products (2^-53,
+// or 2^-24 with FP32 products), plus the rounding of the double-double
+// value (re, im) to FP64.  NaN when m is NaN (the plain kernel keeps none).
+double sweep_error_bound(int n, bool fp32, double m, double re, double im);
+
+// The calling thread's pqp_last_error_bound().
+void capi_set_last_bound(double bound);
+
 }  // namespace pqp

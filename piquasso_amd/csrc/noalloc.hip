@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <limits>
 #include <mutex>
 #include <string>
 
@@ -102,12 +103,13 @@ extern "C" int pqp_permanent_c128_noalloc(const double *matrix, int64_t m_r, int
   int st = plan_build(std::vector<PermInput>{in}, single_plan_options(flags), plan, err);
   if (st != PQP_OK) return capi_fail(st, err);
 
-  // scratch layout: [arena][partials: 4 doubles per slot][results: 2 + 4 doubles]
+  // scratch layout: [arena][partials: 4 doubles per slot][term magnitudes:
+  // 1 double per slot][results: 2 + 4 + 1 doubles]
   const size_t arena_bytes = plan.arena.size();
   const size_t partials_off = align256(arena_bytes);
-  const size_t partials_bytes = static_cast<size_t>(std::max<int64_t>(plan.n_partials, 1)) * 4 * sizeof(double);
-  const size_t results_off = align256(partials_off + partials_bytes);
-  const size_t needed = results_off + 6 * sizeof(double);
+  const size_t n_slots = static_cast<size_t>(std::max<int64_t>(plan.n_partials, 1));
+  const size_t results_off = align256(partials_off + n_slots * 5 * sizeof(double));
+  const size_t needed = results_off + 7 * sizeof(double);
   if (needed > kScratchBytes)
     return capi_fail(PQP_ERR_RANGE, "this permanent needs " + std::to_string(needed) +
                                         " bytes of device scratch; the allocation-free entry "
@@ -125,8 +127,10 @@ extern "C" int pqp_permanent_c128_noalloc(const double *matrix, int64_t m_r, int
   a.items = reinterpret_cast<const ItemDesc *>(base + plan.items_off);
   a.units = reinterpret_cast<const WorkUnit *>(base + plan.units_off);
   a.partials = reinterpret_cast<double *>(base + partials_off);
+  a.part_abs = a.partials + 4 * n_slots;
   a.out = reinterpret_cast<double *>(base + results_off);
   a.out_dd = a.out + 2;
+  a.out_abs = a.out + 6;
   a.queue_head = nullptr;  // static launches use no queue
   a.n_units = static_cast<int32_t>(plan.units.size());
   a.n_items = static_cast<int32_t>(plan.items.size());
@@ -143,10 +147,14 @@ extern "C" int pqp_permanent_c128_noalloc(const double *matrix, int64_t m_r, int
     PQP_NA_HIP(launch_fwd(r.bucket, l));
   }
   if (!plan.all_direct) PQP_NA_HIP(launch_finalize(comp, a, 0, a.n_items, 0, plan.max_units, nullptr));
-  double res[6];
+  double res[7];
   PQP_NA_HIP(hipMemcpy(res, base + results_off, sizeof(res), hipMemcpyDeviceToHost));
   out[0] = res[0];
   out[1] = res[1];
   if (out_dd) std::memcpy(out_dd, res + 2, 4 * sizeof(double));
+  const int n = plan.items.empty() ? 0 : plan.items[0].n;
+  capi_set_last_bound(n == 0 ? 0.0
+                      : comp ? sweep_error_bound(n, plan.fp32, res[6], res[0], res[1])
+                             : std::numeric_limits<double>::quiet_NaN());
   return PQP_OK;
 }

@@ -136,8 +136,9 @@ __device__ __forceinline__ void column_product(const R (&cr)[MCP], const R (&ci)
   pi = ai;
 }
 
-// One wave unit: top states [64*local, 64*local+64) of item `item`.
-template <int MCP, typename R, bool COMP>
+// One wave unit: top states [64*local, 64*local+64) of item `item`.  ABS
+// (compensated static launches): also sum the magnitudes of the terms.
+template <int MCP, typename R, bool COMP, bool ABS>
 __device__ __forceinline__ void sweep_unit(const LaunchArgs &a, int item, int local) {
   const cptr<ItemDesc> it = as_const<ItemDesc>(a.items) + item;
   const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
@@ -181,6 +182,9 @@ __device__ __forceinline__ void sweep_unit(const LaunchArgs &a, int item, int lo
   // read.
   constexpr int PF = MCP < 16 ? MCP : 16;
   dd accr{0.0, 0.0}, acci{0.0, 0.0};
+  // magnitudes of the terms (ABS only): sum_v |coef| (|Re p| + |Im p|),
+  // two FMAs with |.| source modifiers into one accumulator (two VGPRs)
+  double abs_acc = 0.0;
   InnerEntry e = inner_entry(tab, 0);
   for (int v0 = 0; v0 < S; v0 += kDdFold) {
     const int vend = v0 + kDdFold < S ? v0 + kDdFold : S;
@@ -198,6 +202,10 @@ __device__ __forceinline__ void sweep_unit(const LaunchArgs &a, int item, int lo
       column_product<MCP, R>(cr, ci, pr, pi, flags, cmax, ps, cpow);
       sr = __builtin_fma(e.coef, static_cast<double>(pr), sr);
       si = __builtin_fma(e.coef, static_cast<double>(pi), si);
+      if constexpr (ABS) {
+        abs_acc = __builtin_fma(__builtin_fabs(e.coef), __builtin_fabs(static_cast<double>(pr)), abs_acc);
+        abs_acc = __builtin_fma(__builtin_fabs(e.coef), __builtin_fabs(static_cast<double>(pi)), abs_acc);
+      }
       // row sums += (+-2) x inner row: exact doubling, one rounding (FMA)
       const R s2 = in_vgpr(static_cast<R>(e.sign2));
 #pragma unroll
@@ -226,8 +234,13 @@ __device__ __forceinline__ void sweep_unit(const LaunchArgs &a, int item, int lo
   } else {
     lane_sum = cdd{{accr.hi * lane_w, 0.0}, {acci.hi * lane_w, 0.0}};
   }
+  double abs_sum = abs_acc * __builtin_fabs(lane_w);
   if constexpr (COMP) {
     lane_sum = wave_reduce(lane_sum);
+    if constexpr (ABS) {
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) abs_sum += __shfl_xor(abs_sum, off);
+    }
   } else {
     // plain FP64 butterfly
 #pragma unroll
@@ -251,30 +264,33 @@ __device__ __forceinline__ void sweep_unit(const LaunchArgs &a, int item, int lo
         a.out_dd[4 * o + 2] = im_hi;
         a.out_dd[4 * o + 3] = im_lo;
       }
+      if (ABS && a.out_abs) a.out_abs[o] = ldexp(abs_sum, e);
     } else {
       double *p = a.partials + 4 * (it->partial_base + local);
       p[0] = lane_sum.re.hi;
       p[1] = lane_sum.re.lo;
       p[2] = lane_sum.im.hi;
       p[3] = lane_sum.im.lo;
+      if (ABS && a.part_abs) a.part_abs[it->partial_base + local] = abs_sum;
     }
   }
 }
 
-template <int LO, int HI, typename R, bool COMP>
+template <int LO, int HI, typename R, bool COMP, bool ABS>
 __device__ __forceinline__ void dispatch(const LaunchArgs &a, int mcp, int item, int local) {
   if constexpr (LO <= HI) {
     if (mcp == LO) {
-      sweep_unit<LO, R, COMP>(a, item, local);
+      sweep_unit<LO, R, COMP, ABS>(a, item, local);
     } else {
-      dispatch<LO + 1, HI, R, COMP>(a, mcp, item, local);
+      dispatch<LO + 1, HI, R, COMP, ABS>(a, mcp, item, local);
     }
   }
 }
 
 }  // namespace
 
-// Persistent, work-stealing launch: every wave loops over the queue.
+// Persistent, work-stealing launch: every wave loops over the queue.  It
+// keeps no term magnitudes (the widest variant has no registers to spare).
 template <int LO, int HI, typename R, bool COMP>
 __global__ __launch_bounds__(kWave)
 __attribute__((amdgpu_waves_per_eu(waves_for(HI, sizeof(R) == 4), 8))) void perm_persistent_kernel(LaunchArgs a, int first_unit,
@@ -291,7 +307,7 @@ __attribute__((amdgpu_waves_per_eu(waves_for(HI, sizeof(R) == 4), 8))) void perm
     const int item = uniform(wu->item);
     const int local = uniform(wu->local);
     const int mcp = uniform(as_const<ItemDesc>(a.items)[item].mcp);
-    dispatch<LO, HI, R, COMP>(a, mcp, item, local);
+    dispatch<LO, HI, R, COMP, false>(a, mcp, item, local);
     k = __builtin_amdgcn_readfirstlane(next) + static_cast<int>(gridDim.x);
   }
 }
@@ -304,7 +320,7 @@ __attribute__((amdgpu_waves_per_eu(waves_for(HI, sizeof(R) == 4), 8))) void perm
   const int item = uniform(wu->item);
   const int local = uniform(wu->local);
   const int mcp = uniform(as_const<ItemDesc>(a.items)[item].mcp);
-  dispatch<LO, HI, R, COMP>(a, mcp, item, local);
+  dispatch<LO, HI, R, COMP, COMP>(a, mcp, item, local);
 }
 
 // One block of NT threads per item: thread t sums the item's partials
@@ -317,7 +333,7 @@ __attribute__((amdgpu_waves_per_eu(waves_for(HI, sizeof(R) == 4), 8))) void perm
 template <bool COMP, int NT>
 __global__ __launch_bounds__(NT) void perm_finalize_kernel(LaunchArgs a, int item0, int n_heads) {
   constexpr int NW = NT / kWave;
-  __shared__ double part[NW][4];
+  __shared__ double part[NW][5];
   const int item = item0 + static_cast<int>(blockIdx.x);
   const int tid = static_cast<int>(threadIdx.x);
   const int lane = tid & (kWave - 1);
@@ -328,14 +344,17 @@ __global__ __launch_bounds__(NT) void perm_finalize_kernel(LaunchArgs a, int ite
   const int units = it->units;
   const int64_t base = it->partial_base;
   cdd s = cdd_zero();
+  double abs_sum = 0.0;  // magnitudes of the terms (COMP with part_abs)
+  const bool with_abs = COMP && a.part_abs && a.out_abs;
   if (it->n == 0) {
     // empty selection: permanent 1 by definition
-    if (tid == 0) s.re.hi = 1.0;
+    if (tid == 0) s.re.hi = abs_sum = 1.0;
   } else {
     for (int k = tid; k < units; k += NT) {
       const double *p = a.partials + 4 * (base + k);
       if constexpr (COMP) {
         s = cdd_add(s, cdd{{p[0], p[1]}, {p[2], p[3]}});
+        if (with_abs) abs_sum += a.part_abs[base + k];
       } else {
         s.re.hi += p[0];
         s.im.hi += p[2];
@@ -344,6 +363,8 @@ __global__ __launch_bounds__(NT) void perm_finalize_kernel(LaunchArgs a, int ite
   }
   if constexpr (COMP) {
     s = wave_reduce(s);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) abs_sum += __shfl_xor(abs_sum, off);
   } else {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
@@ -357,12 +378,15 @@ __global__ __launch_bounds__(NT) void perm_finalize_kernel(LaunchArgs a, int ite
       part[wave][1] = s.re.lo;
       part[wave][2] = s.im.hi;
       part[wave][3] = s.im.lo;
+      part[wave][4] = abs_sum;
     }
     __syncthreads();
     if (tid == 0) {
       s = cdd{{part[0][0], part[0][1]}, {part[0][2], part[0][3]}};
+      abs_sum = part[0][4];
       for (int w = 1; w < NW; ++w) {
         const cdd o{{part[w][0], part[w][1]}, {part[w][2], part[w][3]}};
+        abs_sum += part[w][4];
         if constexpr (COMP) {
           s = cdd_add(s, o);
         } else {
@@ -385,6 +409,7 @@ __global__ __launch_bounds__(NT) void perm_finalize_kernel(LaunchArgs a, int ite
       a.out_dd[4 * o + 2] = im_hi;
       a.out_dd[4 * o + 3] = im_lo;
     }
+    if (with_abs) a.out_abs[o] = ldexp(abs_sum, e);
   }
 }
 

@@ -118,6 +118,12 @@ struct LaunchArgs {
   int *queue_head;      // dynamic work-queue head (persistent launches)
   int32_t n_units;
   int32_t n_items;
+  // Forward sweeps with double-double accumulation also sum the magnitudes
+  // of their terms, sum_t |coef| |w_top| (|Re p| + |Im p|) >= sum_t |term|:
+  // one double per unit (part_abs) and, scaled by 2^-(n-1), one per
+  // permanent (out_abs).  Null where a launch does not keep them.
+  double *part_abs = nullptr;
+  double *out_abs = nullptr;
 };
 
 }  // namespace pqp

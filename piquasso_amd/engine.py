@@ -46,9 +46,12 @@ def _prep(matrix, rows, cols, dtype=np.complex128):
     return a, r, c
 
 
-def permanent_ex(matrix, rows, cols, mode: str = "dd"):
+def permanent_ex(matrix, rows, cols, mode: str = "dd", with_bound: bool = False):
     """Return ``(value, dd)`` for one permanent of a complex128 matrix:
-    ``value`` the complex result, ``dd`` = [re_hi, re_lo, im_hi, im_lo]."""
+    ``value`` the complex result, ``dd`` = [re_hi, re_lo, im_hi, im_lo];
+    with ``with_bound`` also a bound on |value - permanent|
+    (``pqp_last_error_bound``: the sweep's 4 (n + 2) u sum|terms| / 2^(n-1)
+    plus the final rounding; NaN in mode "plain")."""
     if mode not in MODES:
         raise ValueError(f"mode must be one of {sorted(MODES)}")
     a, r, c = _prep(matrix, rows, cols)
@@ -60,6 +63,8 @@ def permanent_ex(matrix, rows, cols, mode: str = "dd"):
         out.ctypes.data_as(_dp), dd.ctypes.data_as(_dp),
     )
     _capi.check(st)
+    if with_bound:
+        return complex(out[0], out[1]), dd, float(_capi.lib().pqp_last_error_bound())
     return complex(out[0], out[1]), dd
 
 
@@ -230,6 +235,16 @@ class PermanentBatch:
         )
         vals = (out[:, 0] + 1j * out[:, 1])[: self.count]
         return (vals, dd[: self.count]) if with_dd else vals
+
+    def error_bounds(self) -> np.ndarray:
+        """Bounds on |result - permanent| of every item from the last run:
+        4 (n + 2) u sum|terms| / 2^(n-1) plus the final rounding (u = 2^-53,
+        or 2^-24 with FP32 products), from the term magnitudes the sweep
+        kernels sum beside the terms.  NaN where the last run kept none (the
+        plain baseline, gradient batches)."""
+        b = np.zeros(max(self.count, 1))
+        _capi.check(_capi.lib().pqp_batch_error_bounds(self._h, b.ctypes.data_as(_dp)))
+        return b[: self.count]
 
     def gradients(self):
         """d perm / d A of every item from the last run (gradient batches):
