@@ -238,7 +238,9 @@ void pqp_batch_destroy(pqp_batch *b);
  * from the pybind entry permanent_laplace(matrix, rows, cols)
  * (piquasso/_math/permanent.cpp:43-58,82-84).  For a column with cols[l] = 0
  * the value is the uncut sum (the reference computes the same there); with
- * all row multiplicities 0 every entry is 1.  out: 2 * m_c doubles.
+ * all row multiplicities 0 every entry is 1.  out: 2 * m_c doubles.  Up to
+ * 32 columns of non-zero multiplicity one sweep gives every minor; wider
+ * problems (up to 64 particles) run one forward sweep per column.
  */
 int pqp_laplace_c128(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
                      const int32_t *rows, const int32_t *cols, double *out);
@@ -322,9 +324,12 @@ int pqp_perm_poly_plan_info(int64_t n, int64_t n_vars, int64_t n_out, const int3
  * = r_i c_j perm(A_k; r_k - e_i, c_k - e_j) (0 where r_i or c_j is 0), the
  * result of grad_perm (src/permanent.cpp:271-300) and of the JAX extension's
  * backward (src/jax_perm/jax_perm_core.cpp:141-208), here without the
- * square-matrix restriction.  All minors of all permanents run in one
- * launch of column-minor sweeps (one per row of non-zero multiplicity).
- * perms (2 * count doubles, may be null) receives the permanents.
+ * square-matrix restriction.  Up to 32 particles one Gray sweep per
+ * permanent gives the permanent and every entry (perm_grad.hip); larger
+ * permanents (up to 64) take one column-minor sweep per row of non-zero
+ * multiplicity, or, beyond 32 distinct columns, one forward sweep per
+ * (row, column) minor, all in one launch.  perms (2 * count doubles, may be
+ * null) receives the permanents.
  */
 int pqp_grad_batch_c128(int64_t count, const double *const *matrices, const int64_t *m_r,
                         const int64_t *m_c, const int64_t *ld, const int32_t *const *rows,

@@ -436,6 +436,7 @@ struct pqp_batch {
   Exec xm;
   std::vector<std::vector<int32_t>> minor_rows;  // r - e_i of every minor item
   std::vector<int64_t> minor_item, minor_row;    // (permanent, row) of each minor item
+  std::vector<std::vector<int64_t>> minor_slot;  // output slot of each minor (per column)
   std::vector<int64_t> m_r, m_c;                 // shapes of the permanents
   std::vector<std::vector<int32_t>> rows, cols;  // multiplicities of the permanents
   hipGraph_t graph = nullptr;
@@ -852,7 +853,107 @@ int fetch_out(Exec &x, std::vector<double> &res) {
 }
 
 // grad[i, j] = r_i c_j perm(A; r - e_i, c - e_j) from the column-minor results.
-void scatter_gradients(const Plan &mp, const std::vector<double> &res,
+// Column minors perm(A; r, c - e_l), l < m_c, of problems with
+// sum c = sum r + 1, as one plan, and where each lands in its output:
+// slot[q][l] is the complex output index of problem q's minor l.  Problems
+// with at most 32 columns of non-zero multiplicity run on the column-minor
+// kernels (one sweep for all of a problem's minors).  If any problem is
+// wider, every problem runs as forward items instead, one per column: rows
+// r and columns c - e_l, or for c_l = 0 columns c with one column to spare
+// (the uncut product the reference's loop gives such a column,
+// src/permanent_laplace.cpp:168-172); the forward kernels take every width
+// up to 64.
+struct MinorsPlan {
+  Plan plan;
+  std::vector<std::vector<int64_t>> slot;
+};
+
+constexpr int64_t kMinorsMaxCols = 32;
+
+int build_minors_plan(const std::vector<PermInput> &probs, PlanOptions opt, MinorsPlan &mp,
+                      std::string &err) {
+  bool wide = false;
+  for (const PermInput &in : probs) {
+    int64_t nz = 0;
+    if (in.cols)
+      for (int64_t j = 0; j < in.m_c; ++j) nz += in.cols[j] > 0;
+    wide = wide || nz > kMinorsMaxCols;
+  }
+  mp.slot.assign(probs.size(), {});
+  if (!wide) {
+    opt.minors = true;
+    const int st = plan_build(probs, opt, mp.plan, err);
+    if (st != PQP_OK) return st;
+    for (size_t q = 0; q < probs.size(); ++q) {
+      const ItemInfo &inf = mp.plan.info[q];
+      const int64_t mc = static_cast<int64_t>(inf.colsrc.size());
+      // columns of zero multiplicity get the uncut product (slot mc)
+      mp.slot[q].assign(static_cast<size_t>(std::max<int64_t>(probs[q].m_c, 0)), inf.out_offset + mc);
+      for (int64_t l = 0; l < mc; ++l) mp.slot[q][static_cast<size_t>(inf.colsrc[l])] = inf.out_offset + l;
+    }
+    return PQP_OK;
+  }
+  // validate as the minors planner does before expanding into items
+  for (size_t q = 0; q < probs.size(); ++q) {
+    const PermInput &in = probs[q];
+    if (in.m_r < 0 || in.m_c < 0 || (in.m_r * in.m_c > 0 && !in.a) || (in.m_r > 0 && !in.rows) ||
+        (in.m_c > 0 && !in.cols) || (in.m_r > 0 && in.ld < in.m_c)) {
+      err = "item " + std::to_string(q) + ": invalid shape or null pointer";
+      return PQP_ERR_INVALID;
+    }
+    int64_t sr = 0, sc = 0;
+    for (int64_t i = 0; i < in.m_r; ++i) {
+      if (in.rows[i] < 0) {
+        err = "item " + std::to_string(q) + ": negative row multiplicity";
+        return PQP_ERR_INVALID;
+      }
+      sr += in.rows[i];
+    }
+    for (int64_t j = 0; j < in.m_c; ++j) {
+      if (in.cols[j] < 0) {
+        err = "item " + std::to_string(q) + ": negative column multiplicity";
+        return PQP_ERR_INVALID;
+      }
+      sc += in.cols[j];
+    }
+    if (sr + 1 != sc) {
+      err = "column multiplicities must sum to one more than the row multiplicities";
+      return PQP_ERR_SUM_MISMATCH;
+    }
+  }
+  size_t n_items = 0;
+  for (const PermInput &in : probs) n_items += static_cast<size_t>(in.m_c);
+  std::vector<std::vector<int32_t>> cols(n_items);
+  std::vector<PermInput> items;
+  items.reserve(n_items);
+  for (size_t q = 0; q < probs.size(); ++q) {
+    const PermInput &in = probs[q];
+    for (int64_t l = 0; l < in.m_c; ++l) {
+      std::vector<int32_t> &c = cols[items.size()];
+      c.assign(in.cols, in.cols + in.m_c);
+      PermInput it = in;
+      if (c[static_cast<size_t>(l)] > 0) {
+        c[static_cast<size_t>(l)] -= 1;
+        it.col_excess = 0;
+      } else {
+        it.col_excess = 1;
+      }
+      it.cols = c.data();
+      mp.slot[q].push_back(static_cast<int64_t>(items.size()));
+      items.push_back(it);
+    }
+  }
+  opt.minors = false;
+  const int st = plan_build(items, opt, mp.plan, err);
+  if (st != PQP_OK) return st;
+  for (size_t q = 0; q < probs.size(); ++q)
+    for (int64_t &s : mp.slot[q]) s = mp.plan.info[static_cast<size_t>(s)].out_offset;
+  return PQP_OK;
+}
+
+// grad[i, j] = r_i c_j perm(A; r - e_i, c - e_j) from the column minors of
+// the problems (rows r - e_i, columns c), minor q of permanent mitem[q].
+void scatter_gradients(const std::vector<std::vector<int64_t>> &slot, const std::vector<double> &res,
                        const std::vector<int64_t> &mitem, const std::vector<int64_t> &mrow,
                        const std::vector<PermInput> &inputs, double *const *grads) {
   for (size_t k = 0; k < inputs.size(); ++k)
@@ -862,13 +963,12 @@ void scatter_gradients(const Plan &mp, const std::vector<double> &res,
     const PermInput &in = inputs[mitem[q]];
     double *g = grads[mitem[q]];
     if (!g) continue;
-    const ItemInfo &inf = mp.info[q];
     const int64_t i = mrow[q];
     const double ri = static_cast<double>(in.rows[i]);
-    for (size_t l = 0; l < inf.colsrc.size(); ++l) {
-      const int64_t j = inf.colsrc[l];
+    for (int64_t j = 0; j < in.m_c; ++j) {
+      if (in.cols[j] <= 0) continue;
       const double f = ri * static_cast<double>(in.cols[j]);
-      const double *m = res.data() + 2 * (inf.out_offset + static_cast<int64_t>(l));
+      const double *m = res.data() + 2 * slot[q][static_cast<size_t>(j)];
       g[2 * (i * in.m_c + j)] = f * m[0];
       g[2 * (i * in.m_c + j) + 1] = f * m[1];
     }
@@ -1098,6 +1198,38 @@ size_t align256(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
 
 namespace {
 
+// All column minors of the problems (sum c = sum r + 1) into outs[q]
+// (2 m_c doubles each, column order), from the single-call state's stream.
+int run_minors(const std::vector<PermInput> &probs, const PlanOptions &opt, double *const *outs) {
+  MinorsPlan mp;
+  std::string err;
+  int st = build_minors_plan(probs, opt, mp, err);
+  if (st != PQP_OK) return fail(st, err);
+  if ((st = require_device()) != PQP_OK) return st;
+  SingleState &S = single_state();
+  std::lock_guard<std::mutex> lock(S.mu);
+  Exec &x = S.x;
+  if (!S.init) {
+    PQP_HIP(hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking));
+    x.own_stream = true;
+    S.init = true;
+  }
+  x.plan = std::move(mp.plan);
+  index_units(x);
+  x.flags = PQP_FLAG_STATIC;
+  if ((st = upload(x)) != PQP_OK) return st;
+  if ((st = enqueue(x, PQP_RUN_EAGER, nullptr, nullptr)) != PQP_OK) return st;
+  std::vector<double> res;
+  if ((st = fetch_out(x, res)) != PQP_OK) return st;
+  for (size_t q = 0; q < probs.size(); ++q)
+    for (int64_t j = 0; j < probs[q].m_c; ++j) {
+      const double *v = res.data() + 2 * mp.slot[q][static_cast<size_t>(j)];
+      outs[q][2 * j] = v[0];
+      outs[q][2 * j + 1] = v[1];
+    }
+  return PQP_OK;
+}
+
 int grad_batch_minors(const std::vector<PermInput> &inputs, double *perms, double *const *grads) {
   Plan fwd;
   std::string err;
@@ -1110,10 +1242,9 @@ int grad_batch_minors(const std::vector<PermInput> &inputs, double *perms, doubl
   std::vector<PermInput> minputs;
   minor_inputs(inputs, mrows, mitem, mrow, minputs);
   PlanOptions mopt;
-  mopt.minors = true;
   mopt.total_units = fopt.total_units;
-  Plan mplan;
-  st = plan_build(minputs, mopt, mplan, err);
+  MinorsPlan mplan;
+  st = build_minors_plan(minputs, mopt, mplan, err);
   if (st != PQP_OK) return fail(st, err);
   if ((st = require_device()) != PQP_OK) return st;
   SingleState &S = single_state();
@@ -1135,16 +1266,14 @@ int grad_batch_minors(const std::vector<PermInput> &inputs, double *perms, doubl
     std::memcpy(perms, res.data(), inputs.size() * 2 * sizeof(double));
   }
   if (!minputs.empty()) {
-    x.plan = std::move(mplan);
+    x.plan = std::move(mplan.plan);
     index_units(x);
     x.flags = PQP_FLAG_STATIC;
     if ((st = upload(x)) != PQP_OK) return st;
     if ((st = enqueue(x, PQP_RUN_EAGER, nullptr, nullptr)) != PQP_OK) return st;
     if ((st = fetch_out(x, res)) != PQP_OK) return st;
-  } else {
-    x.plan = std::move(mplan);
   }
-  scatter_gradients(x.plan, res, mitem, mrow, inputs, grads);
+  scatter_gradients(mplan.slot, res, mitem, mrow, inputs, grads);
   return PQP_OK;
 }
 
@@ -1267,41 +1396,9 @@ int pqp_laplace_c128(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
   if (!out && m_c > 0) return fail(PQP_ERR_INVALID, "out is null");
   PermInput in{matrix, false, m_r, m_c, ld, rows, cols};
   PlanOptions opt;
-  opt.minors = true;
   const DeviceState &dev = device_state();
   opt.auto_units_cap = static_cast<int64_t>(std::max(dev.cus, 1)) * kSingleUnitsPerCu;
-  Plan plan;
-  std::string err;
-  int st = plan_build(std::vector<PermInput>{in}, opt, plan, err);
-  if (st != PQP_OK) return fail(st, err);
-  if ((st = require_device()) != PQP_OK) return st;
-  SingleState &S = single_state();
-  std::lock_guard<std::mutex> lock(S.mu);
-  Exec &x = S.x;
-  if (!S.init) {
-    PQP_HIP(hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking));
-    x.own_stream = true;
-    S.init = true;
-  }
-  x.plan = std::move(plan);
-  index_units(x);
-  x.flags = PQP_FLAG_STATIC;
-  if ((st = upload(x)) != PQP_OK) return st;
-  if ((st = enqueue(x, PQP_RUN_EAGER, nullptr, nullptr)) != PQP_OK) return st;
-  std::vector<double> res;
-  if ((st = fetch_out(x, res)) != PQP_OK) return st;
-  const ItemInfo &inf = x.plan.info[0];
-  const int mc = static_cast<int>(inf.colsrc.size());
-  for (int64_t j = 0; j < m_c; ++j) {
-    // columns of zero multiplicity get the uncut product (slot mc), as the
-    // reference's loop does for them (src/permanent_laplace.cpp:168-172)
-    int64_t slot = mc;
-    for (int l = 0; l < mc; ++l)
-      if (inf.colsrc[l] == j) slot = l;
-    out[2 * j] = res[2 * (inf.out_offset + slot)];
-    out[2 * j + 1] = res[2 * (inf.out_offset + slot) + 1];
-  }
-  return PQP_OK;
+  return run_minors(std::vector<PermInput>{in}, opt, &out);
 }
 
 int pqp_laplace_batch_c128(int64_t count, const double *const *matrices, const int64_t *m_r,
@@ -1319,47 +1416,10 @@ int pqp_laplace_batch_c128(int64_t count, const double *const *matrices, const i
   }
   if (count == 0) return PQP_OK;
   PlanOptions opt;
-  opt.minors = true;
   const DeviceState &dev = device_state();
-  const int64_t cus = static_cast<int64_t>(std::max(dev.cus, 1));
   // about kBatchUnitsPerCu waves per CU, shared in proportion to cost
-  opt.total_units = cus * kBatchUnitsPerCu;
-  Plan plan;
-  std::string err;
-  int st = plan_build(inputs, opt, plan, err);
-  if (st != PQP_OK) return fail(st, err);
-  if ((st = require_device()) != PQP_OK) return st;
-  SingleState &S = single_state();
-  std::lock_guard<std::mutex> lock(S.mu);
-  Exec &x = S.x;
-  if (!S.init) {
-    PQP_HIP(hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking));
-    x.own_stream = true;
-    S.init = true;
-  }
-  x.plan = std::move(plan);
-  index_units(x);
-  x.flags = PQP_FLAG_STATIC;
-  if ((st = upload(x)) != PQP_OK) return st;
-  if ((st = enqueue(x, PQP_RUN_EAGER, nullptr, nullptr)) != PQP_OK) return st;
-  std::vector<double> res;
-  if ((st = fetch_out(x, res)) != PQP_OK) return st;
-  std::vector<int64_t> slot_of;
-  for (int64_t k = 0; k < count; ++k) {
-    const ItemInfo &inf = x.plan.info[k];
-    const int64_t mc = static_cast<int64_t>(inf.colsrc.size());
-    // columns of zero multiplicity get the uncut product (slot mc), as in
-    // pqp_laplace_c128
-    slot_of.assign(static_cast<size_t>(m_c[k]), mc);
-    for (int64_t l = 0; l < mc; ++l) slot_of[static_cast<size_t>(inf.colsrc[l])] = l;
-    double *o = outs[k];
-    for (int64_t j = 0; j < m_c[k]; ++j) {
-      const double *v = res.data() + 2 * (inf.out_offset + slot_of[static_cast<size_t>(j)]);
-      o[2 * j] = v[0];
-      o[2 * j + 1] = v[1];
-    }
-  }
-  return PQP_OK;
+  opt.total_units = static_cast<int64_t>(std::max(dev.cus, 1)) * kBatchUnitsPerCu;
+  return run_minors(inputs, opt, outs);
 }
 
 int pqp_tensor_permanent_batch_c128(int64_t count, const double *const *matrices, const int64_t *n,
@@ -1671,10 +1731,12 @@ int pqp_batch_create(int64_t count, const void *const *matrices, int is_c64, con
     std::vector<PermInput> minputs;
     minor_inputs(inputs, b->minor_rows, b->minor_item, b->minor_row, minputs);
     PlanOptions mopt;
-    mopt.minors = true;
     mopt.total_units = opt.total_units;
-    st = plan_build(minputs, mopt, b->xm.plan, err);
+    MinorsPlan mp;
+    st = build_minors_plan(minputs, mopt, mp, err);
     if (st != PQP_OK) return fail(st, err);
+    b->xm.plan = std::move(mp.plan);
+    b->minor_slot = std::move(mp.slot);
   }
   if ((st = require_device()) != PQP_OK) return st;
   index_units(b->x);
@@ -1843,7 +1905,7 @@ int pqp_batch_gradients(pqp_batch *b, double *const *grads) {
                                b->rows[k].data(), b->cols[k].data()});
   if (b->grad_minors) {
     if (!b->minor_item.empty() && (st = fetch_out(b->xm, res)) != PQP_OK) return st;
-    scatter_gradients(b->xm.plan, res, b->minor_item, b->minor_row, inputs, grads);
+    scatter_gradients(b->minor_slot, res, b->minor_item, b->minor_row, inputs, grads);
     return PQP_OK;
   }
   if ((st = fetch_out(b->x, res)) != PQP_OK) return st;

@@ -141,7 +141,8 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
       sc += std::max(in.cols[j], 0);
       mc += in.cols[j] > 0;
     }
-    const int64_t extra = opt.minors ? 1 : 0;  // minors: sum(c) = sum(r) + 1
+    // minors: sum(c) = sum(r) + 1; forward items: sum(c) = sum(r) + col_excess
+    const int64_t extra = opt.minors ? 1 : in.col_excess;
     if (sr == 0 || sr + extra != sc || sc > 64) continue;  // reported in pass 2
     // T = prod (rho + 1) with the prepend applied to the first minimal row
     int minelem = 0;
@@ -209,7 +210,11 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
       }
       sc += in.cols[j];
     }
-    if (!opt.minors && sr != sc) {
+    if (!opt.minors && (in.col_excess < 0 || in.col_excess > 1 || (opt.grad && in.col_excess))) {
+      err = "item " + std::to_string(k) + ": invalid column excess";
+      return kErrInvalid;
+    }
+    if (!opt.minors && sr + in.col_excess != sc) {
       err = "Number of input and output states should be equal";
       return kErrSumMismatch;
     }

@@ -34,6 +34,10 @@ struct PermInput {
   int64_t m_r, m_c, ld;
   const int32_t *rows;
   const int32_t *cols;
+  // forward items only: the columns may sum to the rows' sum + col_excess
+  // (1 for the uncut product sum_t s(t) prod_j cs_j^{c_j} that the
+  // reference's column-minor loop gives a zero-multiplicity column)
+  int32_t col_excess = 0;
 };
 
 struct PlanOptions {

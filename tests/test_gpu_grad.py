@@ -111,8 +111,99 @@ def test_laplace_edge_cases(gpu):
     assert np.allclose(got, [a[0, 1], a[0, 0]], rtol=1e-6)
     with pytest.raises(RuntimeError):
         pa.permanent_laplace(np.eye(2), [1, 1], [1, 1])
-    with pytest.raises(ValueError):
-        pa.permanent_laplace(np.ones((1, 40)), [39], np.ones(40, int))
+
+
+def _wide_problem(rng, m_c, zero_column):
+    """Two rows of multiplicity ~n/2 over m_c columns (few Gray terms), all
+    columns of multiplicity 1, or one of 0 and one of 2."""
+    n = m_c - 1
+    a = (rng.standard_normal((2, m_c)) + 1j * rng.standard_normal((2, m_c))) / np.sqrt(m_c)
+    r = np.array([n // 2, n - n // 2])
+    c = np.ones(m_c, int)
+    if zero_column:
+        c[3] = 0
+        c[m_c - 2] = 2
+    return a, r, c
+
+
+def test_laplace_beyond_32_columns(gpu, oracle):
+    """33..64 columns of non-zero multiplicity: more than the column-minor
+    kernels hold, so the minors run as forward items (one per column, the
+    uncut product for a zero column), in single calls and in a batch."""
+    rng = np.random.default_rng(13)
+    probs = [_wide_problem(rng, m_c, z) for m_c, z in ((33, False), (40, True), (48, False), (64, True))]
+    batch = pa.permanent_laplace_batch(*zip(*probs))
+    for (a, r, c), got_b in zip(probs, batch):
+        print(f"laplace width {len(c)}", flush=True)
+        got = pa.permanent_laplace(a, r, c)
+        lap = oracle.laplace_dd(a, r, c)
+        tols = _laplace_tols(a, r, c, lap)
+        assert (np.abs(got - lap) <= tols).all(), (len(c), np.abs(got - lap).max())
+        assert (np.abs(got_b - lap) <= tols).all(), len(c)
+    # a narrow problem in the same batch goes the same way
+    a2 = (rng.standard_normal((3, 5)) + 1j * rng.standard_normal((3, 5))) / 3
+    r2, c2 = np.array([1, 2, 1]), np.array([1, 1, 1, 1, 1])
+    mixed = pa.permanent_laplace_batch([probs[1][0], a2], [probs[1][1], r2], [probs[1][2], c2])
+    lap2 = oracle.laplace_dd(a2, r2, c2)
+    assert (np.abs(mixed[1] - lap2) <= _laplace_tols(a2, r2, c2, lap2)).all()
+
+
+def test_gradient_every_width(gpu, oracle):
+    """n = m_c = 1..64 over two rows of multiplicity ~n/2 (few Gray terms):
+    every single-sweep gradient bucket (1-8, 9-16, 17-19, 20, 21-32) and,
+    beyond 32 particles, the forward-item form, against explicit minors."""
+    rng = np.random.default_rng(15)
+    mats, rows, cols = [], [], []
+    for n in list(range(1, 33)) + [33, 40, 48, 64]:
+        m_r = 1 if n == 1 else 2
+        a = (rng.standard_normal((m_r, n)) + 1j * rng.standard_normal((m_r, n))) / np.sqrt(n)
+        mats.append(a)
+        rows.append(np.array([n]) if m_r == 1 else np.array([n // 2, n - n // 2]))
+        cols.append(np.ones(n, int))
+    # one batch per form: a batch holding any permanent above 32 particles
+    # runs every item in the column-minor / forward-item form
+    grads = pa.permanent_grad_batch(mats[:32], rows[:32], cols[:32])[1]
+    grads += pa.permanent_grad_batch(mats[32:], rows[32:], cols[32:])[1]
+    for a, r, c, g in zip(mats, rows, cols, grads):
+        n = int(np.sum(r))
+        ref = oracle.grad_dd(a, r, c)
+        for i in np.flatnonzero(r):
+            ri = r.copy()
+            ri[i] -= 1
+            for j in range(n):
+                cj = c.copy()
+                cj[j] -= 1
+                tol = r[i] * c[j] * _tol(a, ri, cj, ref[i, j] / (r[i] * c[j]))
+                assert abs(g[i, j] - ref[i, j]) <= tol, (n, i, j, g[i, j], ref[i, j])
+
+
+def test_gradient_beyond_32_particles_and_columns(gpu, oracle):
+    """n = 40 over 40 distinct columns: neither the single-sweep gradient
+    kernels (n <= 32) nor the column-minor kernels (<= 32 columns) hold it,
+    so G_ij = r_i c_j perm(A; r - e_i, c - e_j) runs as forward items; the
+    direct entry and a gradient batch agree with the explicit minors."""
+    rng = np.random.default_rng(14)
+    a, r, c = _wide_problem(rng, 41, False)
+    c = c[:40]
+    a = a[:, :40]
+    r = np.array([20, 20])
+    g = pa.permanent_grad(a, r, c)
+    ref = oracle.grad_dd(a, r, c)
+    with pa.PermanentBatch([a], [r], [c], gradient=True) as b:
+        b.run("eager")
+        gb = b.gradients()[0]
+        pb = b.results()[0]
+    p = complex(pa.permanent(a, r, c))
+    assert abs(pb - p) <= _tol(a, r, c, p)
+    for i in range(2):
+        ri = r.copy()
+        ri[i] -= 1
+        for j in range(40):
+            cj = c.copy()
+            cj[j] -= 1
+            tol = r[i] * c[j] * _tol(a, ri, cj, ref[i, j] / (r[i] * c[j]))
+            assert abs(g[i, j] - ref[i, j]) <= tol, (i, j, g[i, j], ref[i, j])
+            assert abs(gb[i, j] - ref[i, j]) <= tol, (i, j)
 
 
 def test_gradient_reference_vectors(gpu, reference_vectors):
@@ -286,7 +377,5 @@ def test_torch_batched_autograd(gpu):
 def test_gradient_errors(gpu):
     with pytest.raises(RuntimeError):
         pa.permanent_grad(np.eye(2), [1, 1], [2, 1])
-    with pytest.raises(ValueError):
-        pa.permanent_grad(np.ones((2, 40)), [20, 20], np.ones(40, int))  # minors: <= 32 columns
     # an empty selection has a zero gradient
     assert not pa.permanent_grad(np.ones((2, 2)), [0, 0], [0, 0]).any()
