@@ -49,6 +49,9 @@ number that is no permanent; here that raises ``RuntimeError``.
 from __future__ import annotations
 
 import ctypes
+import importlib.util
+import os
+import sysconfig
 import threading
 
 import numpy as np
@@ -83,7 +86,26 @@ def _multiplicities(values, name, length):
 
 
 _c128_call = None
+_fast = None
 _tls = threading.local()
+
+
+def _fast_module():
+    """The CPython binding of pqp_permanent_c128 (lib/_pqp_fast*.so, built
+    beside the library from csrc/fastcall.c), or False when it is absent;
+    the ctypes binding then serves every call."""
+    global _fast
+    if _fast is None:
+        _capi.lib()  # the library itself first: its absence is an error
+        path = os.path.join(os.path.dirname(_capi.LIB_PATH),
+                            "_pqp_fast" + sysconfig.get_config_var("EXT_SUFFIX"))
+        mod = False
+        if os.path.exists(path):
+            spec = importlib.util.spec_from_file_location("_pqp_fast", path)
+            mod = importlib.util.module_from_spec(spec)
+            spec.loader.exec_module(mod)
+        _fast = mod
+    return _fast
 
 
 def _fast_c128():
@@ -116,6 +138,16 @@ def permanent(matrix, rows, cols):
             and rows.ndim == 1 and cols.ndim == 1 and rows.dtype.kind in "iu" and cols.dtype.kind in "iu"):
         # the common call: a C-contiguous complex128 matrix and integer
         # arrays; same result, less marshalling
+        fast = _fast if _fast is not None else _fast_module()
+        if fast:
+            try:
+                v = fast.permanent_c128(matrix, rows, cols)
+            except (TypeError, BufferError, ValueError):
+                v = None  # another layout of the multiplicities: the paths below
+            if v is not None:
+                if type(v) is int:
+                    _capi.check(v)
+                return np.array(v, dtype=np.complex128)
         m_r, m_c = matrix.shape
         if rows.shape[0] != m_r or cols.shape[0] != m_c:
             raise ValueError(f"permanent(): rows / cols have {rows.shape[0]} / {cols.shape[0]} "

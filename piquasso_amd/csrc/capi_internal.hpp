@@ -17,10 +17,24 @@ int capi_fail(int code, const std::string &msg);
 // PQP_OK when a HIP device is usable, else PQP_ERR_DEVICE with the message.
 int capi_require_device();
 
-// Planning options of a single-permanent call with these PQP_FLAG_* flags:
-// about 1.4 sqrt(T) waves (capped per compute unit), one-wave permanents
-// write their result from the sweep kernel.
-PlanOptions single_plan_options(uint32_t flags);
+// Gray terms of one permanent, prod (rho_i + 1) after the prepend (the
+// first minimal non-zero row loses one), as the planner counts them;
+// `in.rows` must be valid (negative entries count as 0).
+double gray_terms(const PermInput &in);
+
+// Single permanents of at most this many Gray terms and at most 16 columns
+// run in one launch of one block (perm_single_block_kernel), planned for at
+// most kSingleBlockUnits waves (8 measured faster than 16 at n = 12 and 14:
+// each lane's start state costs about as much as a few Gray steps, so fewer,
+// longer units do less work); larger ones spread over the device.
+constexpr double kSingleBlockMaxTerms = 16384.0;
+constexpr int kSingleBlockUnits = 8;
+
+// Planning options of a single-permanent call of `in` with these
+// PQP_FLAG_* flags: about 1.4 sqrt(T) waves, capped at 16 for the
+// one-block form and per compute unit otherwise; one-wave permanents write
+// their result from the sweep kernel.
+PlanOptions single_plan_options(uint32_t flags, const PermInput &in);
 
 // Bound on |computed - perm| of a floating-point sweep from its term
 // magnitudes m = sum_t |term_t| / 2^(n-1) (as the kernels accumulate them):

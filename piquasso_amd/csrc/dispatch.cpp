@@ -9,6 +9,17 @@ namespace pqp {
     if (l.fp32) return launch_fwd_b##k##_f(l);             \
     return l.comp ? launch_fwd_b##k##_d(l) : launch_fwd_b##k##_p(l);
 
+hipError_t launch_single_block(int bucket, bool fp32, const SingleArgs &args, hipStream_t stream) {
+  switch (bucket) {
+    case 0:
+      return fp32 ? launch_single_block_b0_f(args, stream) : launch_single_block_b0_d(args, stream);
+    case 1:
+      return fp32 ? launch_single_block_b1_f(args, stream) : launch_single_block_b1_d(args, stream);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
 hipError_t launch_fwd(int bucket, const FwdLaunch &l) {
   if (l.fp32 && !l.comp) return hipErrorInvalidValue;  // FP32 products always use dd
   switch (bucket) {

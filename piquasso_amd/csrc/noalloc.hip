@@ -100,7 +100,8 @@ extern "C" int pqp_permanent_c128_noalloc(const double *matrix, int64_t m_r, int
   const PermInput in{matrix, false, m_r, m_c, ld, rows, cols};
   Plan plan;
   std::string err;
-  int st = plan_build(std::vector<PermInput>{in}, single_plan_options(flags), plan, err);
+  if (m_r < 0 || (m_r > 0 && !rows)) return capi_fail(PQP_ERR_INVALID, "invalid shape or null pointer");
+  int st = plan_build(std::vector<PermInput>{in}, single_plan_options(flags, in), plan, err);
   if (st != PQP_OK) return capi_fail(st, err);
 
   // scratch layout: [arena][partials: 4 doubles per slot][term magnitudes:

@@ -28,6 +28,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 
 #include "dd.hpp"
@@ -153,7 +154,7 @@ __device__ __forceinline__ void sweep_unit(const LaunchArgs &a, int item, int lo
   const bool active = u < U;
   if (!active) u = U - 1;
 
-  const cptr<R> Bin = as_const<R>(a.arena, it->b2_off);  // inner rows of B
+  const cptr<R> Bin = b_rows<R>(a, it) + 2 * MCP;  // inner rows of B (after the prepended row)
   const cptr<InnerEntry> tab = as_const<InnerEntry>(a.arena, it->step_off);
   const cptr<int32_t> cpow = as_const<int32_t>(a.arena, it->cpow_off + 24);
   PowState ps{0u, 0u, 0u, 0};
@@ -323,23 +324,18 @@ __attribute__((amdgpu_waves_per_eu(waves_for(HI, sizeof(R) == 4), 8))) void perm
   dispatch<LO, HI, R, COMP, COMP>(a, mcp, item, local);
 }
 
-// One block of NT threads per item: thread t sums the item's partials
-// t, t + NT, ... (double-double), each wave reduces with a butterfly, and
-// the per-wave sums are added in wave order, so the result does not depend
-// on scheduling; then the 2^-(n-1) scale, the complex128 result and its dd
-// parts.  NT = 64 for batches of small items, 1024 when an item has
-// thousands of units.  Block 0 also rewinds the work-queue heads for the
-// next launch of the same plan.
+// The block's sum of one item's partials: thread t adds partials t, t + NT,
+// ... (double-double), each wave reduces with a butterfly, and the
+// per-wave sums are added in wave order, so the result does not depend on
+// scheduling; then the 2^-(n-1) scale, the complex128 result, its dd parts
+// and the term magnitudes (thread 0 writes them).
 template <bool COMP, int NT>
-__global__ __launch_bounds__(NT) void perm_finalize_kernel(LaunchArgs a, int item0, int n_heads) {
+__device__ __forceinline__ void finalize_item(const LaunchArgs &a, int item) {
   constexpr int NW = NT / kWave;
   __shared__ double part[NW][5];
-  const int item = item0 + static_cast<int>(blockIdx.x);
   const int tid = static_cast<int>(threadIdx.x);
   const int lane = tid & (kWave - 1);
   const int wave = tid / kWave;
-  if (blockIdx.x == 0 && tid < n_heads) a.queue_head[tid] = 0;
-  if (item >= a.n_items) return;
   const cptr<ItemDesc> it = as_const<ItemDesc>(a.items) + item;
   const int units = it->units;
   const int64_t base = it->partial_base;
@@ -410,6 +406,88 @@ __global__ __launch_bounds__(NT) void perm_finalize_kernel(LaunchArgs a, int ite
       a.out_dd[4 * o + 3] = im_lo;
     }
     if (with_abs) a.out_abs[o] = ldexp(abs_sum, e);
+  }
+}
+
+// One block of NT threads per item (finalize_item).  NT = 64 for batches of
+// small items, 1024 when an item has thousands of units.  Block 0 also
+// rewinds the work-queue heads for the next launch of the same plan.
+template <bool COMP, int NT>
+__global__ __launch_bounds__(NT) void perm_finalize_kernel(LaunchArgs a, int item0, int n_heads) {
+  const int item = item0 + static_cast<int>(blockIdx.x);
+  if (blockIdx.x == 0 && static_cast<int>(threadIdx.x) < n_heads) a.queue_head[threadIdx.x] = 0;
+  if (item >= a.n_items) return;
+  finalize_item<COMP, NT>(a, item);
+}
+
+// One small permanent (item 0 of a one-item plan) in one launch, for the
+// single-call path: the block's waves sweep the item's units (wave w: units
+// w, w + W, ...), the block adds their partials as the finalize kernel does
+// (a one-unit item has written its result from the sweep), and thread 0
+// then publishes `seq` to `done` in host-visible memory after a
+// system-scope fence, so the host can poll for the result instead of
+// synchronising the stream.
+//
+// A lone small sweep is latency-bound: every first touch of a table line
+// is a scalar-cache miss, and the sweep's loads depend on each other
+// (descriptor -> table -> row), so misses would queue up one after another.
+// The waves therefore first touch every 64-byte line of the plan's arena
+// (wave w: lines w, w + W, ..., four independent loads in flight per wave)
+// and meet at a barrier, after which the sweep's scalar loads hit.
+//
+// The rows of B may come in the kernel arguments (SingleArgs::b): the
+// argument struct is read through the kernel-argument segment pointer, as
+// scalar loads like the arena.
+template <int LO, int HI, typename R>
+__global__ __launch_bounds__(kSingleBlockThreads) void perm_single_block_kernel(SingleArgs args) {
+  constexpr int W = kSingleBlockThreads / kWave;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+  const uintptr_t b_addr = reinterpret_cast<uintptr_t>(__builtin_amdgcn_kernarg_segment_ptr()) +
+                           offsetof(SingleArgs, b);
+  LaunchArgs a = args.a;
+  const int arena_lines = args.arena_lines;
+  int b_lines = 0;
+  if (args.b_in_args) {
+    a.b_rows = reinterpret_cast<const char *>(b_addr);
+    b_lines = kSingleArgBBytes / 64;
+  }
+  {
+    const cptr<uint32_t> lines = as_const<uint32_t>(a.arena);
+    const cptr<uint32_t> blines = reinterpret_cast<cptr<uint32_t>>(b_addr);
+    // straight-line loads with indices clamped to the last line, consumed
+    // together by one asm statement, so all of them are in flight before
+    // a single wait (the first 4 W lines of the arena and the kernel
+    // arguments' rows of B; larger arenas continue in a loop)
+    const int la = arena_lines - 1, lb = b_lines > 0 ? b_lines - 1 : 0;
+    const cptr<uint32_t> bl = b_lines > 0 ? blines : lines;
+    const int lbx = b_lines > 0 ? lb : la;
+    const uint32_t v0 = lines[16 * min(wave, la)], v1 = lines[16 * min(wave + W, la)];
+    const uint32_t v2 = lines[16 * min(wave + 2 * W, la)], v3 = lines[16 * min(wave + 3 * W, la)];
+    const uint32_t v4 = bl[16 * min(wave, lbx)], v5 = bl[16 * min(wave + W, lbx)];
+    const uint32_t v6 = bl[16 * min(wave + 2 * W, lbx)], v7 = bl[16 * min(wave + 3 * W, lbx)];
+    __asm__ volatile("" ::"s"(v0), "s"(v1), "s"(v2), "s"(v3), "s"(v4), "s"(v5), "s"(v6), "s"(v7));
+    uint32_t t = 0;
+    for (int l = wave + 4 * W; l < arena_lines; l += W) t ^= lines[16 * l];
+    __asm__ volatile("" ::"s"(t));
+  }
+  __syncthreads();
+  const cptr<ItemDesc> it = as_const<ItemDesc>(a.items);
+  const int units = uniform(it->units);
+  const int mcp = uniform(it->mcp);
+  for (int u = wave; u < units; u += W) dispatch<LO, HI, R, true, true>(a, mcp, 0, u);
+  __syncthreads();
+  // the partials' sum: wave 0 alone while they fit its lanes (one butterfly,
+  // no cross-wave exchange), else the whole block
+  if (!(uniform(it->flags) & kDirectOut)) {
+    if (units <= kWave) {
+      if (wave == 0) finalize_item<true, kWave>(a, 0);
+    } else {
+      finalize_item<true, kSingleBlockThreads>(a, 0);
+    }
+  }
+  if (threadIdx.x == 0) {
+    __threadfence_system();
+    __hip_atomic_store(args.done, args.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
@@ -498,6 +576,14 @@ hipError_t PQP_NAME(occupancy_fwd_b)(bool persistent, int *blocks) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(
       blocks, perm_static_kernel<PQP_LO, PQP_HI, PQP_R, PQP_COMP>, kWave, 0);
 }
+
+#if PQP_BUCKET_ID <= 1 && PQP_VARIANT != 1
+hipError_t PQP_NAME(launch_single_block_b)(const SingleArgs &args, hipStream_t stream) {
+  hipLaunchKernelGGL((perm_single_block_kernel<PQP_LO, PQP_HI, PQP_R>), dim3(1),
+                     dim3(kSingleBlockThreads), 0, stream, args);
+  return hipGetLastError();
+}
+#endif
 
 #if PQP_BUCKET_ID == 0 && PQP_VARIANT == 0
 hipError_t launch_finalize(bool comp, const LaunchArgs &a, int item0, int n_items,

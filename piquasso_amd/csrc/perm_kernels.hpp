@@ -41,6 +41,14 @@ PQP_DECLARE_BUCKET(7)
 #undef PQP_DECLARE_BUCKET
 #undef PQP_DECLARE
 
+// One-launch single permanents (perm_single_block_kernel), widths 1..16:
+// d = FP64 + dd, f = FP32 products + dd.
+hipError_t launch_single_block_b0_d(const SingleArgs &args, hipStream_t s);
+hipError_t launch_single_block_b0_f(const SingleArgs &args, hipStream_t s);
+hipError_t launch_single_block_b1_d(const SingleArgs &args, hipStream_t s);
+hipError_t launch_single_block_b1_f(const SingleArgs &args, hipStream_t s);
+hipError_t launch_single_block(int bucket, bool fp32, const SingleArgs &args, hipStream_t stream);
+
 constexpr int kBuckets = 8;        // forward kernel width buckets
 constexpr int kMinorBuckets = 9;   // column-minor kernel buckets
 constexpr int kMaxRanges = 9;      // concurrent launches of one plan (one per bucket)

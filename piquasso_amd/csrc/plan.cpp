@@ -563,12 +563,12 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     if (opt.grad) {
       partial_cursor += static_cast<int64_t>(d.units) * (grad_unit_stride(rows_B, mcp) / 4);
       out_cursor += 1 + static_cast<int64_t>(rows_B) * mc;
-      plan.info[k].rowsrc.assign(src_rows.begin(), src_rows.end());
     } else {
       partial_cursor += static_cast<int64_t>(d.units) * (opt.minors ? mcp + 1 : 1);
       out_cursor += opt.minors ? mc + 1 : 1;
     }
     plan.info[k].colsrc.assign(colsrc.begin(), colsrc.end());
+    plan.info[k].rowsrc.assign(src_rows.begin(), src_rows.end());
     plan.items[k] = d;
 
     ItemInfo &inf = plan.info[k];
@@ -643,6 +643,38 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
   std::memcpy(plan.arena.data() + plan.items_off, plan.items.data(),
               plan.items.size() * sizeof(ItemDesc));
   return kOk;
+}
+
+void plan_refresh_matrix(Plan &plan, const std::vector<PermInput> &inputs) {
+  const size_t rsz = plan.fp32 ? sizeof(float) : sizeof(double);
+  for (size_t k = 0; k < plan.items.size() && k < inputs.size(); ++k) {
+    const ItemInfo &inf = plan.info[k];
+    if (inf.trivial_one) continue;
+    const ItemDesc &d = plan.items[k];
+    const PermInput &in = inputs[k];
+    const int mc = static_cast<int>(inf.colsrc.size());
+    char *base = plan.arena.data() + d.b_off;
+    for (size_t r = 0; r < inf.rowsrc.size(); ++r)
+      for (int j = 0; j < mc; ++j) {
+        const int64_t e = (inf.rowsrc[r] * in.ld + inf.colsrc[static_cast<size_t>(j)]) * 2;
+        double re, im;
+        if (in.a_is_float) {
+          re = static_cast<const float *>(in.a)[e];
+          im = static_cast<const float *>(in.a)[e + 1];
+        } else {
+          re = static_cast<const double *>(in.a)[e];
+          im = static_cast<const double *>(in.a)[e + 1];
+        }
+        const int64_t o = (static_cast<int64_t>(r) * d.mcp + j) * 2;
+        if (rsz == sizeof(float)) {
+          reinterpret_cast<float *>(base)[o] = static_cast<float>(re);
+          reinterpret_cast<float *>(base)[o + 1] = static_cast<float>(im);
+        } else {
+          reinterpret_cast<double *>(base)[o] = re;
+          reinterpret_cast<double *>(base)[o + 1] = im;
+        }
+      }
+  }
 }
 
 }  // namespace pqp

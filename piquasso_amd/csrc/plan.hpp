@@ -69,7 +69,7 @@ struct ItemInfo {
   bool trivial_one = false;     // n == 0: permanent is 1 by definition
   int64_t out_offset = 0;       // first output slot (complex values)
   std::vector<int64_t> colsrc;  // source column of each kernel column
-  std::vector<int64_t> rowsrc;  // gradient items: source row of each row of B
+  std::vector<int64_t> rowsrc;  // source row of each row of B (prepended, inner, top)
 };
 
 struct Plan {
@@ -101,6 +101,12 @@ struct Plan {
 struct Bucket { int id; int lo; int hi; };
 const std::vector<Bucket> &buckets();
 int bucket_for_width(int mc, bool fp32, int *mcp_out);
+
+// Rewrite the rows of B in a forward plan's arena for new matrices of the
+// same shapes and multiplicities as the plan's inputs (every other table
+// depends on the multiplicities alone); the byte range of item k's rows is
+// [items[k].b_off, + info[k].rowsrc.size() * mcp * 2 * (4 or 8)).
+void plan_refresh_matrix(Plan &plan, const std::vector<PermInput> &inputs);
 
 int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
                Plan &plan, std::string &err);

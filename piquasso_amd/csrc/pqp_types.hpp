@@ -23,6 +23,7 @@
 namespace pqp {
 
 constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kSingleBlockThreads = 1024;  // one-launch single permanents: 16 waves in one block
 constexpr int kMaxCols = 64;       // widest row-sum vector a kernel keeps in VGPRs
 constexpr int kMaxDigits = 63;     // Gray digits (rows) per permanent
 constexpr int kDdFold = 64;        // FP64 run length folded into dd accumulators
@@ -124,6 +125,25 @@ struct LaunchArgs {
   // permanent (out_abs).  Null where a launch does not keep them.
   double *part_abs = nullptr;
   double *out_abs = nullptr;
+  // One-item plans only: the rows of B of item 0 at this address instead of
+  // arena + b_off (the copy a one-block launch carries in its kernel
+  // arguments); null everywhere else.
+  const char *b_rows = nullptr;
+};
+
+// Kernel arguments of the one-block single-permanent launch: the launch
+// arguments, the done slot it publishes to, and up to kSingleArgBBytes of
+// the rows of B (when they fit), which then need no copy to the device.
+// The whole struct is the kernel's only argument, so it sits at offset 0
+// of the kernel-argument segment (the first explicit argument's place).
+constexpr int kSingleArgBBytes = 3840;
+struct SingleArgs {
+  LaunchArgs a;
+  int32_t arena_lines;  // 64-byte lines of the arena the block touches first
+  int32_t b_in_args;    // 1: rows of B in b[] (a.b_rows is set in the kernel)
+  uint64_t *done;       // host-visible slot receiving seq
+  uint64_t seq;
+  alignas(64) char b[kSingleArgBBytes];
 };
 
 }  // namespace pqp

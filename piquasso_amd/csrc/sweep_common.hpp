@@ -107,11 +107,18 @@ constexpr int waves_for(int hi, bool fp32) {
 
 // Row sums of lane state u (top state u, inner digits all 0) and the lane's
 // top weight prod_top C(rho_t, g_t).
+// Rows of B of item `it`: in the arena, or in the kernel arguments of a
+// one-block single-permanent launch (a.b_rows).
+template <typename R>
+__device__ __forceinline__ cptr<R> b_rows(const LaunchArgs &a, const cptr<ItemDesc> it) {
+  return a.b_rows ? as_const<R>(a.b_rows) : as_const<R>(a.arena, it->b_off);
+}
+
 template <int MCP, typename R>
 __device__ __forceinline__ void start_state(const LaunchArgs &a, const cptr<ItemDesc> it,
                                             int64_t u, int L, int Dt, int flags,
                                             R (&cr)[MCP], R (&ci)[MCP], double &w_top) {
-  const cptr<R> B = as_const<R>(a.arena, it->b_off);
+  const cptr<R> B = b_rows<R>(a, it);
   const cptr<double> alpha = as_const<double>(a.arena, it->alpha_off);
   // Row sums at the lane's first state,
   //   b_0 + sum_inner rho_k b_k + sum_top (rho_t - 2 g_t(u)) b_t,

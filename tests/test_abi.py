@@ -182,6 +182,26 @@ def test_integer_entry_errors_precede_the_device():
         pa.permanent_exact(np.eye(2, dtype=int), [1, 1], [2, 1])
 
 
+def test_fast_binding_of_the_drop_in_call():
+    """The CPython binding of pqp_permanent_c128 (csrc/fastcall.c) is built
+    beside the library and loaded by permanent(); it reports C-ABI failures
+    as their status codes and refuses other argument layouts with TypeError
+    (permanent() then takes its general path)."""
+    from piquasso_amd._math import permanent as PM
+
+    fast = PM._fast_module()
+    assert fast, "lib/_pqp_fast*.so is not built"
+    a = np.eye(2, dtype=np.complex128)
+    one = np.ones(2, dtype=np.int64)
+    assert fast.permanent_c128(a, one, np.array([2, 1], dtype=np.int32)) == _capi.PQP_ERR_SUM_MISMATCH
+    for bad in ((a.astype(np.complex64), one, one), (a, one.astype(np.float64), one),
+                (a, np.ones(3, dtype=np.int64), one), (np.ones(3, dtype=np.complex128), one, one)):
+        with pytest.raises(TypeError):
+            fast.permanent_c128(*bad)
+    with pytest.raises(ValueError):  # numpy's buffer export refuses a non-contiguous view
+        fast.permanent_c128(np.eye(3, dtype=np.complex128)[:, :2], np.ones(3, np.int64), one)
+
+
 def test_empty_selection_needs_no_device():
     """perm of an empty selection is 1 (src/permanent.cpp:106-108); the
     Laplace entry returns the reference's [1] for degenerate inputs."""
