@@ -167,14 +167,17 @@ def times_linear(poly, const, lin, out):
     return out
 
 
-def _ps_probability(u, dist, modes, remaining):
-    """sampling.py:532-551 (in-place polynomial updates)."""
+def _ps_probability(u, dist, modes, remaining, in_place=True):
+    """sampling.py:532-551.  in_place: the reference's in-place polynomial
+    updates (each factor's shifted terms read values already scaled by the
+    constant); False: the product with a fresh copy per factor, the exact
+    probability that the distinguishable particles supply ``remaining``."""
     poly = np.zeros(tuple(np.asarray(remaining) + 1))
     poly[(0,) * len(remaining)] = 1.0
     for mode, mult in enumerate(dist):
         p = np.abs(u[modes, mode]) ** 2
         for _ in range(int(mult)):
-            times_linear(poly, 1.0 - p.sum(), p, out=poly)
+            poly = times_linear(poly, 1.0 - p.sum(), p, out=poly if in_place else np.empty_like(poly))
     return poly[tuple(remaining)]
 
 
@@ -201,7 +204,9 @@ def _ps_conditioned(u, dist, modes, remaining, table, rng):
         fut = nxt[tuple(left)]
         pf = np.abs(u[free, mode]) ** 2
         pp = np.abs(u[list(modes), mode]) ** 2
-        ploss = 1.0 - pf.sum() - pp.sum()
+        # clamped at 0 like the product sampler: the reference's choice()
+        # raises when rounding leaves this below 0, and is unchanged otherwise
+        ploss = max(1.0 - pf.sum() - pp.sum(), 0.0)
         wp = np.zeros(len(modes))
         for a in range(len(modes)):
             if left[a] > 0:
@@ -220,7 +225,7 @@ def _ps_conditioned(u, dist, modes, remaining, table, rng):
 
 
 def _one_sample_postselected_overlap(d, occ, u, rng, reject, laplace, modes, photons, max_trials,
-                                     overlap):
+                                     overlap, in_place=True):
     """sampling.py:422-529: split, one postselection attempt for the
     indistinguishable particles, accept the distinguishable ones with their
     (in-place evaluated) postselection probability, place them conditioned;
@@ -239,7 +244,7 @@ def _one_sample_postselected_overlap(d, occ, u, rng, reject, laplace, modes, pho
         except SamplingAborted:
             continue
         remaining = photons - ind[modes]
-        if rng.random() > _ps_probability(u, dist, modes, remaining):
+        if rng.random() > _ps_probability(u, dist, modes, remaining, in_place):
             continue
         table = _ps_table(u, dist, modes, photons)
         out = _ps_conditioned(u, dist, modes, remaining, table, rng) + ind
@@ -250,11 +255,14 @@ def generate_samples(input_occupation, shots: int, interferometer, seed: int,
                      reject_condition: Optional[Callable[[], bool]] = None,
                      postselect: Sequence = ((), (), 1000),
                      uniform_particle_overlap: Optional[float] = None,
-                     laplace=laplace_dd):
+                     laplace=laplace_dd, ps_acceptance_in_place: bool = True):
     """Samples of ``generate_samples`` (sampling.py:34-107) for the ideal,
     uniformly lossy (``reject_condition``), postselected and uniformly
     partially distinguishable cases, and postselection with uniform partial
-    distinguishability; one shot after another."""
+    distinguishability; one shot after another.  ``ps_acceptance_in_place``
+    False evaluates the distinguishable particles' postselection probability
+    exactly instead of in place as the reference does (the product sampler
+    deliberately does the former)."""
     occ = np.asarray(input_occupation, dtype=int)
     d = len(occ)
     n = int(occ.sum())
@@ -275,7 +283,8 @@ def generate_samples(input_occupation, shots: int, interferometer, seed: int,
             if len(ps_modes):
                 s = _one_sample_postselected_overlap(d, occ, u, rng, reject, laplace, ps_modes,
                                                      ps_photons, max_trials,
-                                                     float(uniform_particle_overlap))
+                                                     float(uniform_particle_overlap),
+                                                     ps_acceptance_in_place)
                 samples.append(tuple(int(v) for v in s))
                 continue
             # sampling.py:328-368

@@ -121,6 +121,7 @@ def test_sampler_matches_oracle_sampler(gpu, variant):
         kw_o["postselect"] = ((0, 5), (1, 1), 1000)
     if variant == "postselect_overlap":
         kw_p["uniform_particle_overlap"] = kw_o["uniform_particle_overlap"] = 0.7
+        kw_o["ps_acceptance_in_place"] = False  # the product's exact acceptance
         u = 0.9 * u  # lossy: the distinguishable particles' loss weight stays positive
     print(f"sampler vs oracle: {variant}, d={d} n={sum(occ)} shots={shots}", flush=True)
     if variant == "lossy":

@@ -135,6 +135,9 @@ def test_lockstep_host_logic_matches_oracle(lockstep_on_oracle, variant):
         kw_p["postselect_data"] = ((1, 4), (1, 0), 500)
         kw_o["postselect"] = ((1, 4), (1, 0), 500)
     if "postselect_overlap" in variant:
+        # the product evaluates the distinguishable particles' postselection
+        # probability exactly, not in place as the reference does
+        kw_o["ps_acceptance_in_place"] = False
         # the distinguishable particles' loss weight is 1 - sum |T|^2, which
         # rounding can make slightly negative for a unitary (numpy's choice
         # then raises, in the reference as here): a lossy T
@@ -191,6 +194,18 @@ def test_in_place_linear_factor_semantics():
     p2[0, 0] = 1.0
     times_linear(p2, 0.5, [0.2, 0.3], out=p2)
     assert np.allclose(p2, [[0.5, 0.5 * 0.3], [0.5 * 0.2, 0.5 * 0.2 * 0.3]])
+    # so the reference's acceptance probability for one distinguishable
+    # particle reaching a postselected mode with probability 0.3 is 0.21; the
+    # product sampler (and the oracle with in_place=False) uses 0.3, and for
+    # two particles the exact 2 p (1 - p)
+    from oracle.sampling import _ps_probability
+    import piquasso_amd.sampling as P
+
+    u = np.array([[np.sqrt(0.7), np.sqrt(0.3)], [np.sqrt(0.3), -np.sqrt(0.7)]], dtype=complex)
+    assert np.isclose(_ps_probability(u, [1, 0], [1], np.array([1])), 0.7 * 0.3)
+    assert np.isclose(_ps_probability(u, [1, 0], [1], np.array([1]), in_place=False), 0.3)
+    assert np.isclose(P._dist_postselection_probability(u, [1, 0], [1], np.array([1])), 0.3)
+    assert np.isclose(P._dist_postselection_probability(u, [2, 0], [1], np.array([1])), 2 * 0.3 * 0.7)
 
 
 def test_postselected_overlap_reference_support(lockstep_on_oracle):
