@@ -603,7 +603,7 @@ def main():
             from piquasso_amd.tune import best_config
 
             _, cus, arch = _capi.device_info()
-            cfg32 = best_config(1, cus, arch) or {"persistent": False, "total_units": 128 * cus}
+            cfg32 = best_config(1, cus, arch, log2_terms=31.0) or {"persistent": False, "total_units": 128 * cus}
             b = pa.PermanentBatch([a32], [r32], [c32], **cfg32)
         ms = b.time(args.replay, reps=args.steps)
         print(json.dumps({"profile": args.profile, "ms": ms.tolist()}))
@@ -668,7 +668,7 @@ def main():
     # measured for n = 32 on MI355X)
     from piquasso_amd.tune import best_config
 
-    cfg32 = best_config(1, cus, arch) or {"persistent": False, "total_units": 128 * cus}
+    cfg32 = best_config(1, cus, arch, log2_terms=31.0) or {"persistent": False, "total_units": 128 * cus}
     b32 = pa.PermanentBatch([a32], [r32], [c32], **cfg32)
     b32.run("eager")
     v32 = b32.results()[0]

@@ -155,11 +155,11 @@ class PermanentBatch:
             if persistent is None:
                 persistent = False
         elif persistent is None or not total_units:
-            from .tune import best_config
+            from .tune import best_config, mean_log2_terms
 
             try:
                 _, cus, arch = _capi.device_info()
-                tuned = best_config(len(matrices), cus, arch)
+                tuned = best_config(len(matrices), cus, arch, log2_terms=mean_log2_terms(rows))
             except RuntimeError:  # no device: creation fails below with the C library's error
                 tuned = {}
             if persistent is None:

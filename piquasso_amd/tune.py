@@ -79,23 +79,63 @@ GRIDS = {
 }
 
 
-def best_config(count: int, cus: int, arch: str, path: str = None) -> dict:
-    """Launch settings for a batch of `count` permanents from the tuned table
-    of this device (PermanentBatch kwargs: total_units, persistent), or {}
-    when no table was written for it.  One permanent uses the single-n32
-    entry, a batch the batched_uneven entry; unit counts scale with the CU
-    count of the table's device."""
+# log2 of the mean Gray terms per item of each tuned workload class (the
+# tuner writes the same figure into the table as "log2_terms"; these are the
+# fallback for tables written before it did)
+CLASS_LOG2_TERMS = {"single_n32": 31.0, "single_n24": 23.0, "batched_uneven": 14.94,
+                    "uniform_n12_20": 16.84}
+SINGLE_CLASSES = ("single_n24", "single_n32")
+BATCH_CLASSES = ("batched_uneven", "uniform_n12_20")
+
+
+def choose_class(table: dict, count: int, log2_terms=None) -> str:
+    """The tuned workload class for `count` permanents whose mean Gray term
+    count is 2^log2_terms: among the single-permanent classes for one
+    permanent, the batch classes otherwise, the one of nearest log2 term
+    count that the table holds (the first present one when log2_terms is
+    None); None when the table holds none of them."""
+    wl = table.get("workloads", {})
+    names = [k for k in (SINGLE_CLASSES if count == 1 else BATCH_CLASSES) if k in wl]
+    if not names:
+        return None
+    if log2_terms is None:
+        return names[-1] if count == 1 else names[0]
+
+    def dist(name):
+        return abs(float(wl[name].get("log2_terms", CLASS_LOG2_TERMS[name])) - float(log2_terms))
+
+    return min(names, key=dist)
+
+
+def best_config(count: int, cus: int, arch: str, path: str = None, log2_terms=None) -> dict:
+    """Launch settings for a batch of `count` permanents of mean Gray term
+    count 2^log2_terms from the tuned table of this device (PermanentBatch
+    kwargs: total_units, persistent), or {} when no table was written for
+    it.  The entry is the tuned class of nearest size (choose_class); unit
+    counts scale with the CU count of the table's device."""
     table = load_tuning(path or DEFAULT_TABLE)
     if not table or not arch.split(":")[0] == str(table.get("device", "")).split(":")[0]:
         return {}
-    name = "single_n32" if count == 1 else "batched_uneven"
-    rec = table.get("workloads", {}).get(name)
-    if not rec:
+    name = choose_class(table, count, log2_terms)
+    if name is None:
         return {}
+    rec = table["workloads"][name]
     cfg = dict(rec["best"])
     if "total_units" in cfg and table.get("cus"):
         cfg["total_units"] = int(round(cfg["total_units"] / table["cus"] * cus))
     return {k: v for k, v in cfg.items() if k in ("total_units", "persistent")}
+
+
+def mean_log2_terms(rows, sample: int = 256) -> float:
+    """log2 of the mean Gray term count over (up to `sample` evenly spaced)
+    items' row multiplicities."""
+    from .workloads import glynn_terms
+
+    n = len(rows)
+    if n == 0:
+        return 0.0
+    idx = np.unique(np.linspace(0, n - 1, min(n, sample)).astype(int))
+    return float(np.log2(np.mean([float(glynn_terms(rows[k])) for k in idx])))
 
 
 def tune(out: str = DEFAULT_TABLE, reps: int = 5, only=None, log=print) -> dict:
@@ -111,6 +151,7 @@ def tune(out: str = DEFAULT_TABLE, reps: int = 5, only=None, log=print) -> dict:
         best = recs[0]
         table["workloads"][name] = {
             "best": best["config"], "ms": best["ms"], "tflops": best["tflops"],
+            "log2_terms": mean_log2_terms(rows),
             "units_per_cu": best["config"].get("total_units", 0) / cus,
             "all": [{"config": r["config"], "ms": r["ms"]} for r in recs],
         }
