@@ -133,21 +133,24 @@ def permanent(matrix, rows, cols):
     """Permanent of ``matrix`` with row multiplicities ``rows`` and column
     multiplicities ``cols`` (Glynn's formula over the reflected n-ary Gray
     code, src/permanent.cpp:49-264), evaluated on the GPU."""
-    if (type(matrix) is np.ndarray and matrix.dtype == np.complex128 and matrix.ndim == 2
-            and matrix.flags.c_contiguous and type(rows) is np.ndarray and type(cols) is np.ndarray
-            and rows.ndim == 1 and cols.ndim == 1 and rows.dtype.kind in "iu" and cols.dtype.kind in "iu"):
-        # the common call: a C-contiguous complex128 matrix and integer
-        # arrays; same result, less marshalling
+    if type(matrix) is np.ndarray:
+        # the common call, a C-contiguous 2-D complex128 matrix with int32 or
+        # int64 multiplicity arrays, through the CPython binding (which
+        # refuses every other layout with TypeError / ValueError)
         fast = _fast if _fast is not None else _fast_module()
         if fast:
             try:
                 v = fast.permanent_c128(matrix, rows, cols)
             except (TypeError, BufferError, ValueError):
-                v = None  # another layout of the multiplicities: the paths below
+                v = None  # another layout: the paths below
             if v is not None:
                 if type(v) is int:
                     _capi.check(v)
                 return np.array(v, dtype=np.complex128)
+    if (type(matrix) is np.ndarray and matrix.dtype == np.complex128 and matrix.ndim == 2
+            and matrix.flags.c_contiguous and type(rows) is np.ndarray and type(cols) is np.ndarray
+            and rows.ndim == 1 and cols.ndim == 1 and rows.dtype.kind in "iu" and cols.dtype.kind in "iu"):
+        # the same call without the binding: addresses through ctypes
         m_r, m_c = matrix.shape
         if rows.shape[0] != m_r or cols.shape[0] != m_c:
             raise ValueError(f"permanent(): rows / cols have {rows.shape[0]} / {cols.shape[0]} "

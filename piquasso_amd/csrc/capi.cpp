@@ -851,10 +851,8 @@ int run_single(const PermInput &in, uint32_t flags, double *out2, double *out_dd
   const int n = d.n;
   const bool fp32 = x.plan.fp32;
   double *res = static_cast<double *>(x.results_m.host);
-  int bucket = -1, mcp = 0;
-  if (d.units > 0) bucket = bucket_for_width(d.mcp, fp32, &mcp);
-  const bool one_block = !(flags & PQP_FLAG_PLAIN) && d.units > 0 && bucket >= 0 && bucket <= 1 &&
-                         d.units <= kSingleBlockThreads / kWave;
+  int bucket = -1;
+  const bool one_block = single_block_eligible(x.plan, flags, &bucket);
   const size_t b_bytes = x.plan.info[0].rowsrc.size() * static_cast<size_t>(d.mcp) * 2 *
                          (fp32 ? sizeof(float) : sizeof(double));
   const bool b_in_args = one_block && b_bytes <= static_cast<size_t>(kSingleArgBBytes);
@@ -872,12 +870,9 @@ int run_single(const PermInput &in, uint32_t flags, double *out2, double *out_dd
     // of B travel in the kernel arguments when they fit
     volatile uint64_t *done = reinterpret_cast<volatile uint64_t *>(res + 7 * x.n_results);
     thread_local SingleArgs args;
-    args.a = x.args();
-    args.arena_lines = static_cast<int32_t>(x.plan.arena.size() / 64);  // whole lines only
-    args.b_in_args = b_in_args ? 1 : 0;
-    if (b_in_args) std::memcpy(args.b, x.plan.arena.data() + d.b_off, b_bytes);
-    args.done = reinterpret_cast<uint64_t *>(static_cast<double *>(x.results_m.dev) + 7 * x.n_results);
-    args.seq = ++S.xs_seq;
+    single_block_args(x.plan, x.args(),
+                      reinterpret_cast<uint64_t *>(static_cast<double *>(x.results_m.dev) + 7 * x.n_results),
+                      ++S.xs_seq, args);
     *done = 0;
     x.abs_valid = true;
     PQP_HIP(launch_single_block(bucket, fp32, args, x.stream));
@@ -1366,6 +1361,28 @@ int grad_batch_minors(const std::vector<PermInput> &inputs, double *perms, doubl
 namespace pqp {
 
 int capi_fail(int code, const std::string &msg) { return fail(code, msg); }
+
+bool single_block_eligible(const Plan &plan, uint32_t flags, int *bucket) {
+  if (plan.items.size() != 1 || (flags & PQP_FLAG_PLAIN)) return false;
+  const ItemDesc &d = plan.items[0];
+  int mcp = 0;
+  const int b = d.units > 0 ? bucket_for_width(d.mcp, plan.fp32, &mcp) : -1;
+  if (bucket) *bucket = b;
+  return d.units > 0 && b >= 0 && b <= 1 && d.units <= kSingleBlockThreads / kWave;
+}
+
+void single_block_args(const Plan &plan, const LaunchArgs &a, uint64_t *done, uint64_t seq,
+                       SingleArgs &args) {
+  const ItemDesc &d = plan.items[0];
+  const size_t b_bytes = plan.info[0].rowsrc.size() * static_cast<size_t>(d.mcp) * 2 *
+                         (plan.fp32 ? sizeof(float) : sizeof(double));
+  args.a = a;
+  args.arena_lines = static_cast<int32_t>(plan.arena.size() / 64);  // whole lines only
+  args.b_in_args = b_bytes <= static_cast<size_t>(kSingleArgBBytes) ? 1 : 0;
+  if (args.b_in_args) std::memcpy(args.b, plan.arena.data() + d.b_off, b_bytes);
+  args.done = done;
+  args.seq = seq;
+}
 
 double sweep_error_bound(int n, bool fp32, double m, double re, double im) {
   const double u = fp32 ? 0x1p-24 : 0x1p-53;

@@ -8,6 +8,7 @@
 #include <string>
 
 #include "plan.hpp"
+#include "pqp_types.hpp"
 
 namespace pqp {
 
@@ -35,6 +36,17 @@ constexpr int kSingleBlockUnits = 8;
 // one-block form and per compute unit otherwise; one-wave permanents write
 // their result from the sweep kernel.
 PlanOptions single_plan_options(uint32_t flags, const PermInput &in);
+
+// Whether the one-item plan runs as one block (perm_single_block_kernel):
+// compensated (no PQP_FLAG_PLAIN), at least one unit, at most 16 columns
+// and kSingleBlockThreads / 64 units; `bucket` receives its width bucket.
+bool single_block_eligible(const Plan &plan, uint32_t flags, int *bucket);
+
+// The one-block launch's arguments for the one-item plan: `a` with the
+// rows of B copied into the arguments when they fit (else read from the
+// arena), the arena's whole 64-byte lines, and the done slot and value.
+void single_block_args(const Plan &plan, const LaunchArgs &a, uint64_t *done, uint64_t seq,
+                       SingleArgs &args);
 
 // Bound on |computed - perm| of a floating-point sweep from its term
 // magnitudes m = sum_t |term_t| / 2^(n-1) (as the kernels accumulate them):

@@ -18,9 +18,10 @@
 // this library serialise on a mutex; callers that drive the same array from
 // several threads through other means would race.
 //
-// The kernels are those of pqp_permanent_c128 (static launches, one block
-// per work unit, then the finalize kernel), on the null stream, with the
-// same plan, so the results are bit-identical to that entry point's.
+// The kernels are those of pqp_permanent_c128 for the same plan (the
+// one-block launch for small permanents, else static launches of one block
+// per work unit and the finalize kernel), on the null stream, so the
+// results are bit-identical to that entry point's.
 #include <hip/hip_runtime.h>
 #include <rocprofiler-sdk-roctx/roctx.h>
 
@@ -105,12 +106,13 @@ extern "C" int pqp_permanent_c128_noalloc(const double *matrix, int64_t m_r, int
   if (st != PQP_OK) return capi_fail(st, err);
 
   // scratch layout: [arena][partials: 4 doubles per slot][term magnitudes:
-  // 1 double per slot][results: 2 + 4 + 1 doubles]
+  // 1 double per slot][results: 2 + 4 + 1 doubles, then the done slot of a
+  // one-block launch]
   const size_t arena_bytes = plan.arena.size();
   const size_t partials_off = align256(arena_bytes);
   const size_t n_slots = static_cast<size_t>(std::max<int64_t>(plan.n_partials, 1));
   const size_t results_off = align256(partials_off + n_slots * 5 * sizeof(double));
-  const size_t needed = results_off + 7 * sizeof(double);
+  const size_t needed = results_off + 8 * sizeof(double);
   if (needed > kScratchBytes)
     return capi_fail(PQP_ERR_RANGE, "this permanent needs " + std::to_string(needed) +
                                         " bytes of device scratch; the allocation-free entry "
@@ -136,18 +138,29 @@ extern "C" int pqp_permanent_c128_noalloc(const double *matrix, int64_t m_r, int
   a.n_units = static_cast<int32_t>(plan.units.size());
   a.n_items = static_cast<int32_t>(plan.items.size());
   const bool comp = !(flags & PQP_FLAG_PLAIN) || plan.fp32;
-  for (const Plan::Range &r : plan.ranges) {
-    FwdLaunch l;
-    l.args = a;
-    l.first_unit = r.first;
-    l.n_units = r.count;
-    l.persistent = false;
-    l.fp32 = plan.fp32;
-    l.comp = comp;
-    l.stream = nullptr;
-    PQP_NA_HIP(launch_fwd(r.bucket, l));
+  int bucket = -1;
+  if (single_block_eligible(plan, flags, &bucket)) {
+    // the kernel pqp_permanent_c128 runs for this plan (same sums, same
+    // order: bit-identical results); its done slot is scratch here, and the
+    // synchronous copy below waits for the launch
+    static thread_local SingleArgs args;
+    single_block_args(plan, a, reinterpret_cast<uint64_t *>(a.out + 7), 0, args);
+    PQP_NA_HIP(launch_single_block(bucket, plan.fp32, args, nullptr));
+  } else {
+    for (const Plan::Range &r : plan.ranges) {
+      FwdLaunch l;
+      l.args = a;
+      l.first_unit = r.first;
+      l.n_units = r.count;
+      l.persistent = false;
+      l.fp32 = plan.fp32;
+      l.comp = comp;
+      l.stream = nullptr;
+      PQP_NA_HIP(launch_fwd(r.bucket, l));
+    }
+    if (!plan.all_direct)
+      PQP_NA_HIP(launch_finalize(comp, a, 0, a.n_items, 0, plan.max_units, nullptr));
   }
-  if (!plan.all_direct) PQP_NA_HIP(launch_finalize(comp, a, 0, a.n_items, 0, plan.max_units, nullptr));
   double res[7];
   PQP_NA_HIP(hipMemcpy(res, base + results_off, sizeof(res), hipMemcpyDeviceToHost));
   out[0] = res[0];

@@ -137,10 +137,19 @@ __device__ __forceinline__ void column_product(const R (&cr)[MCP], const R (&ci)
   pi = ai;
 }
 
+// A unit's sum over its 64 lanes (every lane holds it after the butterfly)
+// and the magnitudes of its terms.
+struct WaveTotal {
+  cdd s;
+  double abs;
+};
+
 // One wave unit: top states [64*local, 64*local+64) of item `item`.  ABS
 // (compensated static launches): also sum the magnitudes of the terms.
+// `total`, when given, receives the unit's sum as well.
 template <int MCP, typename R, bool COMP, bool ABS>
-__device__ __forceinline__ void sweep_unit(const LaunchArgs &a, int item, int local) {
+__device__ __forceinline__ void sweep_unit(const LaunchArgs &a, int item, int local,
+                                           WaveTotal *total = nullptr) {
   const cptr<ItemDesc> it = as_const<ItemDesc>(a.items) + item;
   const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
   const int64_t U = it->U;
@@ -250,6 +259,10 @@ __device__ __forceinline__ void sweep_unit(const LaunchArgs &a, int item, int lo
       lane_sum.im.hi += __shfl_xor(lane_sum.im.hi, off);
     }
   }
+  if (total) {
+    total->s = lane_sum;
+    total->abs = abs_sum;
+  }
   if (lane == 0) {
     if (flags & kDirectOut) {
       // the item's only unit: scale by 2^-(n-1) and write the result
@@ -278,12 +291,13 @@ __device__ __forceinline__ void sweep_unit(const LaunchArgs &a, int item, int lo
 }
 
 template <int LO, int HI, typename R, bool COMP, bool ABS>
-__device__ __forceinline__ void dispatch(const LaunchArgs &a, int mcp, int item, int local) {
+__device__ __forceinline__ void dispatch(const LaunchArgs &a, int mcp, int item, int local,
+                                         WaveTotal *total = nullptr) {
   if constexpr (LO <= HI) {
     if (mcp == LO) {
-      sweep_unit<LO, R, COMP, ABS>(a, item, local);
+      sweep_unit<LO, R, COMP, ABS>(a, item, local, total);
     } else {
-      dispatch<LO + 1, HI, R, COMP, ABS>(a, mcp, item, local);
+      dispatch<LO + 1, HI, R, COMP, ABS>(a, mcp, item, local, total);
     }
   }
 }
@@ -422,11 +436,10 @@ __global__ __launch_bounds__(NT) void perm_finalize_kernel(LaunchArgs a, int ite
 
 // One small permanent (item 0 of a one-item plan) in one launch, for the
 // single-call path: the block's waves sweep the item's units (wave w: units
-// w, w + W, ...), the block adds their partials as the finalize kernel does
-// (a one-unit item has written its result from the sweep), and thread 0
-// then publishes `seq` to `done` in host-visible memory after a
-// system-scope fence, so the host can poll for the result instead of
-// synchronising the stream.
+// w, w + W, ...), the block adds their sums (a one-unit item has written
+// its result from the sweep), and thread 0 then publishes `seq` to `done`
+// (host-visible memory) after a system-scope fence, so the host can poll
+// for the result instead of synchronising the stream.
 //
 // A lone small sweep is latency-bound: every first touch of a table line
 // is a scalar-cache miss, and the sweep's loads depend on each other
@@ -474,16 +487,45 @@ __global__ __launch_bounds__(kSingleBlockThreads) void perm_single_block_kernel(
   const cptr<ItemDesc> it = as_const<ItemDesc>(a.items);
   const int units = uniform(it->units);
   const int mcp = uniform(it->mcp);
-  for (int u = wave; u < units; u += W) dispatch<LO, HI, R, true, true>(a, mcp, 0, u);
-  __syncthreads();
-  // the partials' sum: wave 0 alone while they fit its lanes (one butterfly,
-  // no cross-wave exchange), else the whole block
-  if (!(uniform(it->flags) & kDirectOut)) {
-    if (units <= kWave) {
-      if (wave == 0) finalize_item<true, kWave>(a, 0);
-    } else {
-      finalize_item<true, kSingleBlockThreads>(a, 0);
+  if (units <= W) {
+    // one unit per wave: the waves' totals meet in LDS and thread 0 adds
+    // them in unit order (a one-unit item has written its result itself)
+    __shared__ double part[W][5];
+    WaveTotal total{cdd_zero(), 0.0};
+    if (wave < units) dispatch<LO, HI, R, true, true>(a, mcp, 0, wave, &total);
+    if ((threadIdx.x & (kWave - 1)) == 0 && wave < units) {
+      part[wave][0] = total.s.re.hi;
+      part[wave][1] = total.s.re.lo;
+      part[wave][2] = total.s.im.hi;
+      part[wave][3] = total.s.im.lo;
+      part[wave][4] = total.abs;
     }
+    __syncthreads();
+    if (threadIdx.x == 0 && !(uniform(it->flags) & kDirectOut)) {
+      cdd s = cdd{{part[0][0], part[0][1]}, {part[0][2], part[0][3]}};
+      double abs_sum = part[0][4];
+      for (int w = 1; w < units; ++w) {
+        s = cdd_add(s, cdd{{part[w][0], part[w][1]}, {part[w][2], part[w][3]}});
+        abs_sum += part[w][4];
+      }
+      const int e = -(it->n - 1);
+      const double re_hi = ldexp(s.re.hi, e), re_lo = ldexp(s.re.lo, e);
+      const double im_hi = ldexp(s.im.hi, e), im_lo = ldexp(s.im.lo, e);
+      const int o = it->out_index;
+      a.out[2 * o] = re_hi + re_lo;
+      a.out[2 * o + 1] = im_hi + im_lo;
+      if (a.out_dd) {
+        a.out_dd[4 * o] = re_hi;
+        a.out_dd[4 * o + 1] = re_lo;
+        a.out_dd[4 * o + 2] = im_hi;
+        a.out_dd[4 * o + 3] = im_lo;
+      }
+      if (a.out_abs) a.out_abs[o] = ldexp(abs_sum, e);
+    }
+  } else {
+    for (int u = wave; u < units; u += W) dispatch<LO, HI, R, true, true>(a, mcp, 0, u);
+    __syncthreads();
+    if (!(uniform(it->flags) & kDirectOut)) finalize_item<true, kSingleBlockThreads>(a, 0);
   }
   if (threadIdx.x == 0) {
     __threadfence_system();
