@@ -537,7 +537,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-child", type=float, default=None)
-    ap.add_argument("--profile", choices=["batch", "n32", "grad", "sampling", "tensor", "lossy_pd"],
+    ap.add_argument("--profile", choices=["batch", "n32", "grad", "sampling", "tensor", "lossy_pd",
+                                          "single12", "exact32"],
                     default=None,
                     help="profiling runs: only launch this workload's kernels --steps times")
     ap.add_argument("--replay", choices=["eager", "graph"], default="eager",
@@ -580,6 +581,20 @@ def main():
             for _ in range(args.steps):
                 pa.lossy_detection_probabilities(outs, t, inp, g)
             print(json.dumps({"profile": "lossy_pd", "modes": d, "photons": n}))
+            return 0
+        if args.profile == "single12":
+            # back-to-back drop-in calls of the n = 12 single configuration
+            a, r, c, _ = _load_extended("plain_single_n12")
+            for _ in range(args.steps):
+                pa.permanent(a, r, c)
+            print(json.dumps({"profile": "single12", "calls": args.steps}))
+            return 0
+        if args.profile == "exact32":
+            # the exact integer permanent of the 32 x 32 all-ones matrix (32!)
+            one = np.ones(32, dtype=np.int64)
+            vals = [pa.permanent_exact(np.ones((32, 32), dtype=np.int64), one, one)
+                    for _ in range(args.steps)]
+            print(json.dumps({"profile": "exact32", "exact": all(v == math.factorial(32) for v in vals)}))
             return 0
         if args.profile == "sampling":
             from piquasso_amd import workloads as W

@@ -146,25 +146,43 @@ __global__ __launch_bounds__(kWave) void perm_int_kernel(LaunchArgs a, IntArgs i
   int_dispatch<MCP>(a, ia, item, local);
 }
 
-// One block per item: thread k < K adds the units' residues of prime k in
-// unit order, multiplies by 2^-(n-1) mod p and writes the residue in [0, p).
-__global__ __launch_bounds__(kWave) void perm_int_finalize_kernel(LaunchArgs a, IntArgs ia) {
+// One block of kIntFinalizeThreads per item: for each prime, the threads
+// add strided subsets of the units' residues, the block reduces them
+// through LDS (modular sums are exact, so the order does not matter), and
+// thread 0 multiplies by 2^-(n-1) mod p and writes the residue in [0, p).
+constexpr int kIntFinalizeThreads = 256;
+
+__global__ __launch_bounds__(kIntFinalizeThreads) void perm_int_finalize_kernel(LaunchArgs a,
+                                                                               IntArgs ia) {
+  __shared__ double red[kIntFinalizeThreads];
   const int item = static_cast<int>(blockIdx.x);
-  const int k = static_cast<int>(threadIdx.x);
-  if (item >= a.n_items || k >= ia.n_primes) return;
+  const int tid = static_cast<int>(threadIdx.x);
+  if (item >= a.n_items) return;
   const cptr<ItemDesc> it = as_const<ItemDesc>(a.items) + item;
   const cptr<double> mod = as_const<double>(a.arena, ia.mod_off);
-  const double p = mod[k], pinv = mod[kIntMaxPrimes + k], inv2 = mod[2 * kIntMaxPrimes + k];
-  double s = 0.0;
-  if (it->n == 0) {
-    s = 1.0;
-  } else {
-    const double *part = a.partials + static_cast<int64_t>(kIntMaxPrimes) * it->partial_base;
-    for (int u = 0; u < it->units; ++u) s = reduce_mod(s + part[static_cast<int64_t>(kIntMaxPrimes) * u + k], p, pinv);
-    s = reduce_mod(mul_mod(s, inv2, p, pinv), p, pinv);
-    if (s < 0.0) s += p;
+  const int units = it->units;
+  const double *part = a.partials + static_cast<int64_t>(kIntMaxPrimes) * it->partial_base;
+  for (int k = 0; k < ia.n_primes; ++k) {
+    const double p = mod[k], pinv = mod[kIntMaxPrimes + k], inv2 = mod[2 * kIntMaxPrimes + k];
+    double s = 0.0;
+    for (int u = tid; u < units; u += kIntFinalizeThreads)
+      s = reduce_mod(s + part[static_cast<int64_t>(kIntMaxPrimes) * u + k], p, pinv);
+    red[tid] = s;
+    __syncthreads();
+    for (int w = kIntFinalizeThreads / 2; w > 0; w >>= 1) {
+      if (tid < w) red[tid] = reduce_mod(red[tid] + red[tid + w], p, pinv);
+      __syncthreads();
+    }
+    if (tid == 0) {
+      double r = 1.0;  // empty selection: 1
+      if (it->n > 0) {
+        r = reduce_mod(mul_mod(red[0], inv2, p, pinv), p, pinv);
+        if (r < 0.0) r += p;
+      }
+      a.out[static_cast<int64_t>(kIntMaxPrimes) * it->out_index + k] = r;
+    }
+    __syncthreads();
   }
-  a.out[static_cast<int64_t>(kIntMaxPrimes) * it->out_index + k] = s;
 }
 
 hipError_t launch_int(int mcp, const LaunchArgs &a, const IntArgs &ia, int first_unit,
@@ -194,7 +212,8 @@ hipError_t launch_int(int mcp, const LaunchArgs &a, const IntArgs &ia, int first
 hipError_t launch_int_finalize(const LaunchArgs &a, const IntArgs &ia, int n_items,
                                hipStream_t stream) {
   if (n_items <= 0) return hipSuccess;
-  hipLaunchKernelGGL(perm_int_finalize_kernel, dim3(n_items), dim3(kWave), 0, stream, a, ia);
+  hipLaunchKernelGGL(perm_int_finalize_kernel, dim3(n_items), dim3(kIntFinalizeThreads), 0, stream,
+                     a, ia);
   return hipGetLastError();
 }
 
