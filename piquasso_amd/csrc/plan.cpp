@@ -12,6 +12,7 @@ namespace {
 
 constexpr int64_t kMaxTerms = int64_t(1) << 62;
 constexpr int64_t kMaxInner = int64_t(1) << 16;   // step/coef table length cap
+constexpr int kXcds = 8;                           // MI355X: 8 XCDs, one L2 each
 constexpr int64_t kGradMaxStates = 128;           // inner states per lane of a gradient unit
 constexpr int kExpandAutoMax = 0;                 // auto mode: never (measured slower)
 
@@ -634,6 +635,38 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
       plan.bucket_of_unit.push_back(b);
     }
     plan.ranges.back().count += static_cast<int32_t>(item_units[k]);
+  }
+  // XCD-aware order within each range: one-block-per-unit launches hand
+  // block b to XCD b mod kXcds, and each XCD has its own L2.  Cut every
+  // item's units into chunks, deal the chunks in LPT order to the XCD list
+  // with the fewest units, and interleave the lists, so that the units of
+  // one chunk run on one XCD and its tables are fetched into one L2 instead
+  // of eight (approximately LPT order overall; only the tail, where lists
+  // run out unevenly, leaves the mapping).
+  {
+    std::vector<WorkUnit> lists[kXcds];
+    std::vector<WorkUnit> tmp;
+    for (Plan::Range &r : plan.ranges) {
+      const int64_t chunk = std::max<int64_t>(1, r.count / (kXcds * 16));
+      for (auto &l : lists) l.clear();
+      size_t fill[kXcds] = {};
+      for (int32_t u = r.first; u < r.first + r.count;) {
+        // a run of one item's units, at most `chunk` long
+        int32_t v = u + 1;
+        while (v < r.first + r.count && v - u < chunk && plan.units[v].item == plan.units[u].item) ++v;
+        int x = 0;
+        for (int y = 1; y < kXcds; ++y)
+          if (fill[y] < fill[x]) x = y;
+        lists[x].insert(lists[x].end(), plan.units.begin() + u, plan.units.begin() + v);
+        fill[x] += static_cast<size_t>(v - u);
+        u = v;
+      }
+      tmp.clear();
+      for (size_t round = 0; tmp.size() < static_cast<size_t>(r.count); ++round)
+        for (int x = 0; x < kXcds; ++x)
+          if (round < lists[x].size()) tmp.push_back(lists[x][round]);
+      std::copy(tmp.begin(), tmp.end(), plan.units.begin() + r.first);
+    }
   }
   plan.units_off = arena_alloc(plan.arena,
                                int64_t(std::max<size_t>(plan.units.size(), 1)) * sizeof(WorkUnit));
