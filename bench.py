@@ -140,7 +140,7 @@ def cpu_baseline_child(seconds: float) -> None:
     singles = {}
     for cid in SINGLE_CONFIGS:
         a, r, c, _ = _load_extended(cid)
-        reps = 50 if a.shape[0] <= 12 else 5
+        reps = 200 if a.shape[0] <= 12 else 5
         singles[cid] = _median_call_us(lambda: fn(a, r, c), reps)
     print(json.dumps({"kind": kind, "items": done, "seconds": dt, "flops": flops,
                       "threads": int(os.environ.get("OMP_THREAD_LIMIT", "0") or 0),
@@ -737,7 +737,7 @@ def main():
     singles = {}
     for cid, what in SINGLE_CONFIGS.items():
         a, r, c, ref_c = _load_extended(cid)
-        reps = 50 if a.shape[0] <= 12 else 20
+        reps = 200 if a.shape[0] <= 12 else 20
         got = complex(pa.permanent(a, r, c))
         rec_s = {"what": what, "gpu_call_us": _median_call_us(lambda: pa.permanent(a, r, c), reps),
                  "rel_err": abs(got - ref_c) / abs(ref_c)}
