@@ -56,6 +56,12 @@ extern "C" {
 #define PQP_FLAG_KEEP_COLS 0x10u
 /* Batches: also evaluate d perm / d A for every permanent (pqp_batch_gradients). */
 #define PQP_FLAG_GRADIENT 0x20u
+/* The planner evaluates perm(A; r, c) as perm(A^T; c, r), with the columns
+ * as Gray digits, when that sweep is clearly cheaper (fewer Gray terms, or
+ * unit multiplicities on the product side); the two are equal.
+ * PQP_FLAG_AS_GIVEN keeps the rows as the digits, as the reference does,
+ * for results that round as the reference's orientation does. */
+#define PQP_FLAG_AS_GIVEN 0x80u
 
 /* Library version string. */
 const char *pqp_version(void);

@@ -30,6 +30,7 @@ PQP_FLAG_STATIC = 0x4
 PQP_FLAG_EXPAND_COLS = 0x8
 PQP_FLAG_KEEP_COLS = 0x10
 PQP_FLAG_GRADIENT = 0x20
+PQP_FLAG_AS_GIVEN = 0x80
 PQP_PLAN_GRADIENT_SWEEP = 0x40
 
 PQP_RUN_EAGER = 0

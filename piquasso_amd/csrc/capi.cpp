@@ -684,7 +684,7 @@ int run_int(const PermInput &in, const IntScan &sc, uint64_t *v) {
       a[static_cast<size_t>(2 * (i * in.m_c + j))] =
           in.a_is_float ? static_cast<const float *>(in.a)[2 * (i * in.ld + j)]
                         : static_cast<const double *>(in.a)[2 * (i * in.ld + j)];
-  PlanOptions opt = single_plan_options(0, in);
+  PlanOptions opt = single_plan_options(PQP_FLAG_AS_GIVEN, in);
   opt.direct_out = false;
   opt.expand_cols = 1;  // unit column multiplicities: the product is over n factors
   opt.pad8 = true;
@@ -1413,6 +1413,7 @@ PlanOptions single_plan_options(uint32_t flags, const PermInput &in) {
   PlanOptions opt;
   opt.fp32_products = (flags & PQP_FLAG_FP32) != 0;
   opt.expand_cols = expand_mode(flags);
+  opt.orient = (flags & PQP_FLAG_AS_GIVEN) == 0;
   const DeviceState &dev = device_state();
   // waves per permanent: ~1.4 sqrt(T), at most kSingleUnitsPerCu per CU
   // (16 for the one-block form); a permanent of one wave writes its result
@@ -1469,7 +1470,8 @@ int pqp_permanent_c64(const float *matrix, int64_t m_r, int64_t m_c, int64_t ld,
   PermInput in{matrix, true, m_r, m_c, ld, rows, cols};
   double o[2];
   const int st =
-      run_single(in, PQP_FLAG_FP32 | (flags & (PQP_FLAG_EXPAND_COLS | PQP_FLAG_KEEP_COLS)), o, out_dd);
+      run_single(in, PQP_FLAG_FP32 | (flags & (PQP_FLAG_EXPAND_COLS | PQP_FLAG_KEEP_COLS | PQP_FLAG_AS_GIVEN)),
+                 o, out_dd);
   if (st != PQP_OK) return st;
   out[0] = static_cast<float>(o[0]);
   out[1] = static_cast<float>(o[1]);
@@ -1824,6 +1826,7 @@ int pqp_batch_create(int64_t count, const void *const *matrices, int is_c64, con
   PlanOptions opt;
   opt.fp32_products = (flags & PQP_FLAG_FP32) != 0 || is_c64 != 0;
   opt.expand_cols = expand_mode(flags);
+  opt.orient = (flags & PQP_FLAG_AS_GIVEN) == 0;
   opt.target_inner = target_inner;
   // about kBatchUnitsPerCu waves per CU over the whole batch, split by cost
   opt.total_units = total_units > 0
@@ -2042,6 +2045,7 @@ int pqp_plan_image(const double *matrix, int64_t m_r, int64_t m_c, int64_t ld,
   PlanOptions opt;
   opt.fp32_products = (flags & PQP_FLAG_FP32) != 0;
   opt.expand_cols = expand_mode(flags);
+  opt.orient = (flags & PQP_FLAG_AS_GIVEN) == 0;
   opt.target_inner = target_inner;
   opt.min_units = min_units;
   opt.minors = (flags & PQP_FLAG_GRADIENT) != 0;

@@ -103,8 +103,84 @@ void gray_steps(const std::vector<int> &radix, std::vector<int> &digit,
   }
 }
 
-int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
+namespace {
+
+// The caller's row stride covers the view's rows (its columns when the
+// view is transposed).
+bool stride_ok(const PermInput &in) {
+  return in.transposed ? (in.m_c == 0 || in.ld >= in.m_r) : (in.m_r == 0 || in.ld >= in.m_c);
+}
+
+// Modelled cost of the Glynn sweep with `rows` as Gray digits and `cols`
+// as the product side: T = prod (rho_i + 1) after the prepend, times the
+// work per term (row updates, column product with grouped powers or the
+// count form, accumulation), the grouped form weighted by the lower issue
+// rate of its per-column snapshot tests (about 76% VALU issue against 94%
+// for unit multiplicities, measured on MI355X).
+double sweep_cost(const int32_t *rows, int64_t m_r, const int32_t *cols, int64_t m_c) {
+  int minelem = 0;
+  int64_t i0 = -1;
+  for (int64_t i = 0; i < m_r; ++i)
+    if (rows[i] > 0 && (minelem == 0 || rows[i] < minelem)) {
+      minelem = rows[i];
+      i0 = i;
+    }
+  double T = 1.0;
+  for (int64_t i = 0; i < m_r; ++i) {
+    const int rho = rows[i] - (i == i0 ? 1 : 0);
+    if (rho > 0) T *= rho + 1;
+  }
+  int mc = 0, cmax = 0, n = 0;
+  for (int64_t j = 0; j < m_c; ++j)
+    if (cols[j] > 0) {
+      ++mc;
+      cmax = std::max(cmax, static_cast<int>(cols[j]));
+      n += cols[j];
+    }
+  if (cmax <= 1) return T * (2.0 * mc + 6.0 * std::max(mc - 1, 0) + 4.0);
+  const bool grouped = mc <= kPowGroupedMaxCols && cmax <= 4;
+  const double mult = grouped ? mc + cmax - 2 : n - 1;
+  return T * (2.0 * mc + 6.0 * mult + 4.0) * 1.25;
+}
+
+// Items whose transposed sweep costs less than 0.9 of the given one are
+// planned transposed: perm(A; r, c) = perm(A^T; c, r), the permanent of the
+// expanded matrix being invariant under transposition.  Invalid items are
+// left alone (the planner reports them).
+void orient_items(const std::vector<PermInput> &in, std::vector<PermInput> &out, bool &any) {
+  any = false;
+  out = in;
+  for (PermInput &v : out) {
+    if (v.col_excess || v.m_r <= 0 || v.m_c <= 0 || !v.a || !v.rows || !v.cols || !stride_ok(v))
+      continue;
+    int64_t sr = 0, sc = 0;
+    bool neg = false;
+    for (int64_t i = 0; i < v.m_r; ++i) {
+      neg = neg || v.rows[i] < 0;
+      sr += v.rows[i];
+    }
+    for (int64_t j = 0; j < v.m_c; ++j) {
+      neg = neg || v.cols[j] < 0;
+      sc += v.cols[j];
+    }
+    if (neg || sr != sc || sr == 0 || sr > kMaxCols) continue;
+    if (sweep_cost(v.cols, v.m_c, v.rows, v.m_r) < 0.9 * sweep_cost(v.rows, v.m_r, v.cols, v.m_c)) {
+      std::swap(v.m_r, v.m_c);
+      std::swap(v.rows, v.cols);
+      v.transposed = true;
+      any = true;
+    }
+  }
+}
+
+}  // namespace
+
+int plan_build(const std::vector<PermInput> &caller_inputs, const PlanOptions &opt,
                Plan &plan, std::string &err) {
+  std::vector<PermInput> oriented;
+  bool any_transposed = false;
+  if (opt.orient && !opt.minors && !opt.grad) orient_items(caller_inputs, oriented, any_transposed);
+  const std::vector<PermInput> &inputs = any_transposed ? oriented : caller_inputs;
   plan = Plan();
   plan.fp32 = opt.fp32_products;
   const int64_t n_items = static_cast<int64_t>(inputs.size());
@@ -131,8 +207,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
   for (int64_t k = 0; k < n_items; ++k) {
     const PermInput &in = inputs[k];
     if (in.m_r < 0 || in.m_c < 0 || (in.m_r * in.m_c > 0 && !in.a) ||
-        (in.m_r > 0 && !in.rows) || (in.m_c > 0 && !in.cols) ||
-        (in.m_r > 0 && in.ld < in.m_c)) {
+        (in.m_r > 0 && !in.rows) || (in.m_c > 0 && !in.cols) || !stride_ok(in)) {
       err = "item " + std::to_string(k) + ": invalid shape or null pointer";
       return kErrInvalid;
     }
@@ -191,8 +266,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
       item_min_units = std::max<int64_t>(item_min_units, static_cast<int64_t>(std::ceil(share)));
     }
     if (in.m_r < 0 || in.m_c < 0 || (in.m_r * in.m_c > 0 && !in.a) ||
-        (in.m_r > 0 && !in.rows) || (in.m_c > 0 && !in.cols) ||
-        (in.m_r > 0 && in.ld < in.m_c)) {
+        (in.m_r > 0 && !in.rows) || (in.m_c > 0 && !in.cols) || !stride_ok(in)) {
       err = "item " + std::to_string(k) + ": invalid shape or null pointer";
       return kErrInvalid;
     }
@@ -423,7 +497,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
       }
     };
     auto entry = [&](int64_t row, int64_t col, double &re, double &im) {
-      const int64_t e = (row * in.ld + col) * 2;
+      const int64_t e = (in.transposed ? col * in.ld + row : row * in.ld + col) * 2;
       if (in.a_is_float) {
         const float *p = static_cast<const float *>(in.a);
         re = p[e];
@@ -570,6 +644,7 @@ int plan_build(const std::vector<PermInput> &inputs, const PlanOptions &opt,
     }
     plan.info[k].colsrc.assign(colsrc.begin(), colsrc.end());
     plan.info[k].rowsrc.assign(src_rows.begin(), src_rows.end());
+    plan.info[k].transposed = in.transposed;
     plan.items[k] = d;
 
     ItemInfo &inf = plan.info[k];
@@ -689,7 +764,8 @@ void plan_refresh_matrix(Plan &plan, const std::vector<PermInput> &inputs) {
     char *base = plan.arena.data() + d.b_off;
     for (size_t r = 0; r < inf.rowsrc.size(); ++r)
       for (int j = 0; j < mc; ++j) {
-        const int64_t e = (inf.rowsrc[r] * in.ld + inf.colsrc[static_cast<size_t>(j)]) * 2;
+        const int64_t rs = inf.rowsrc[r], cs = inf.colsrc[static_cast<size_t>(j)];
+        const int64_t e = (inf.transposed ? cs * in.ld + rs : rs * in.ld + cs) * 2;
         double re, im;
         if (in.a_is_float) {
           re = static_cast<const float *>(in.a)[e];

@@ -38,6 +38,10 @@ struct PermInput {
   // (1 for the uncut product sum_t s(t) prod_j cs_j^{c_j} that the
   // reference's column-minor loop gives a zero-multiplicity column)
   int32_t col_excess = 0;
+  // set by the planner only: this is the transposed view of the caller's
+  // input (m_r / m_c and rows / cols swapped; entry (i, j) is the caller's
+  // (j, i), `ld` still the caller's row stride)
+  bool transposed = false;
 };
 
 struct PlanOptions {
@@ -59,6 +63,8 @@ struct PlanOptions {
   bool grad = false;            // gradient items: the permanent and d perm / d B for every
                                 // row of B from one sweep (perm_grad.hip); outputs
                                 // 1 + rows_B * m_c values per item
+  bool orient = true;           // forward items: plan perm(A^T; c, r) instead of
+                                // perm(A; r, c) when its sweep is clearly cheaper
 };
 
 // Per-item bookkeeping kept on the host.
@@ -70,6 +76,7 @@ struct ItemInfo {
   int64_t out_offset = 0;       // first output slot (complex values)
   std::vector<int64_t> colsrc;  // source column of each kernel column
   std::vector<int64_t> rowsrc;  // source row of each row of B (prepended, inner, top)
+  bool transposed = false;      // planned as perm(A^T; c, r): rowsrc are columns of A, colsrc rows
 };
 
 struct Plan {

@@ -37,6 +37,19 @@ def _expected_bound(a, r, c, u=EPS):
     return 4 * (n + 2) * u * glynn_abs_sum(a, r, c)
 
 
+def _matches_a_sweep(bound, a, r, c):
+    """True when `bound` (less its final rounding) is the magnitude bound of
+    the sweep as given or of the transposed sweep perm(A^T; c, r), whichever
+    the planner ran."""
+    for ar, rr, cc in ((np.asarray(a), r, c), (np.asarray(a).T, c, r)):
+        if W.glynn_terms(rr) > 1 << 22:
+            continue  # far too many terms for the planner to choose, or for the CPU sum
+        lo = _expected_bound(ar, rr, cc)
+        if lo * (1 - 1e-9) <= bound <= math.sqrt(2) * lo * (1 + 1e-9):
+            return True
+    return False
+
+
 def test_single_bounds_hold_and_match_the_term_magnitudes(gpu, oracle):
     rng = np.random.default_rng(31)
     for trial in range(40):
@@ -48,10 +61,7 @@ def test_single_bounds_hold_and_match_the_term_magnitudes(gpu, oracle):
         got, _, bound = pa.permanent_ex(a, r, c, with_bound=True)
         ref = oracle.perm_dd(a, r, c)
         assert abs(got - ref) <= bound, (trial, abs(got - ref), bound)
-        lo = _expected_bound(a, r, c)
-        rounding = EPS * abs(got)
-        assert lo * (1 - 1e-9) <= bound - rounding <= math.sqrt(2) * lo * (1 + 1e-9), \
-            (trial, bound, lo)
+        assert _matches_a_sweep(bound - EPS * abs(got), a, r, c), (trial, bound)
 
 
 def test_every_kernel_width_within_its_bound(gpu, oracle):
@@ -70,8 +80,7 @@ def test_every_kernel_width_within_its_bound(gpu, oracle):
         vals, bounds = b.results(), b.error_bounds()
     for k in range(64):
         assert abs(vals[k] - refs[k]) <= bounds[k], (k + 1, abs(vals[k] - refs[k]), bounds[k])
-        lo = _expected_bound(mats[k], rows[k], cols[k])
-        assert lo * (1 - 1e-9) <= bounds[k] - EPS * abs(vals[k]) <= math.sqrt(2) * lo * (1 + 1e-9), k
+        assert _matches_a_sweep(bounds[k] - EPS * abs(vals[k]), mats[k], rows[k], cols[k]), k
     for k in (0, 15, 31, 47, 63):
         # a single call splits the sweep into other units: same terms, sums
         # in another order

@@ -144,6 +144,25 @@ def test_plan_splits_large_items(oracle):
     assert abs(emulate(buf) - ref) <= 1e-11 * abs(ref)
 
 
+def test_orientation_choice(oracle):
+    """perm(A; r, c) = perm(A^T; c, r): six unit rows against two columns of
+    multiplicity 3 sweep 2^5 Gray terms as given and 4 x 4 / 2 transposed, so
+    the planner takes the transposed form (the six rows of A become the
+    product's columns) unless PQP_FLAG_AS_GIVEN keeps the rows as digits."""
+    rng = np.random.default_rng(8)
+    a = rng.standard_normal((6, 2)) + 1j * rng.standard_normal((6, 2))
+    r, c = np.ones(6, int), np.array([3, 3])
+    ref = oracle.perm_dd(a, r, c)
+    for flags, mc in ((0, 6), (_capi.PQP_FLAG_AS_GIVEN, 2)):
+        buf = plan_image(a, r, c, flags=flags)
+        d = np.frombuffer(buf[:128].tobytes(), dtype=ITEM)[0]
+        assert d["mc"] == mc, flags
+        assert abs(emulate(buf) - ref) <= 1e-12 * abs(ref), flags
+    # a square matrix with unit multiplicities keeps its orientation
+    d = np.frombuffer(plan_image(a[:2], [1, 1], [1, 1])[:128].tobytes(), dtype=ITEM)[0]
+    assert d["mc"] == 2
+
+
 def test_column_expansion_layout():
     a = np.arange(6, dtype=float).reshape(2, 3) + 0j
     r, c = [2, 3], [3, 0, 2]
@@ -266,7 +285,8 @@ def test_grouped_power_masks():
     assert d["flags"] & 8
     # sorted multiplicities 3 3 2 1: drops d = 0 1 1 0
     assert masks == [0b0110, 0, 0]
-    buf = plan_image(a, [4, 3, 2], [6, 1, 1, 1])
+    # kept as given: planned transposed, the rows 4 3 2 would be the product side
+    buf = plan_image(a, [4, 3, 2], [6, 1, 1, 1], flags=_capi.PQP_FLAG_AS_GIVEN)
     d = np.frombuffer(buf[:128].tobytes(), dtype=ITEM)[0]
     assert not d["flags"] & 8  # drop of 5 > 3: count form
 
