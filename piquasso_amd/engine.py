@@ -130,6 +130,10 @@ class PermanentBatch:
         also evaluate d perm / d A of every item on each run (column-minor
         sweeps after the permanents, on the same stream and in the same
         graph); read them with :meth:`gradients`.  complex128 / "fp64" only.
+    as_given:
+        sweep every item with its rows as the Gray digits, as the reference
+        does.  By default the planner sweeps perm(A^T; c, r) instead where
+        that is clearly cheaper (the same number, rounded differently).
     """
 
     def __init__(
@@ -143,6 +147,7 @@ class PermanentBatch:
         expand_cols=None,
         total_units: int = 0,
         gradient: bool = False,
+        as_given: bool = False,
     ):
         if not (len(matrices) == len(rows) == len(cols)):
             raise ValueError("matrices, rows and cols must have the same length")
@@ -198,6 +203,8 @@ class PermanentBatch:
             flags |= _capi.PQP_FLAG_KEEP_COLS
         if gradient:
             flags |= _capi.PQP_FLAG_GRADIENT
+        if as_given:
+            flags |= _capi.PQP_FLAG_AS_GIVEN
         handle = ctypes.c_void_p()
         L = _capi.lib()
         st = L.pqp_batch_create(
