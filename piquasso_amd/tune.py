@@ -72,7 +72,7 @@ def _workloads():
 
 
 GRIDS = {
-    "batched_uneven": {"total_units": [4096, 8192, 12288, 16384], "persistent": [False, True]},
+    "batched_uneven": {"total_units": [4096, 5120, 6144, 8192, 12288], "persistent": [False, True]},
     "single_n32": {"total_units": [16384, 32768, 65536], "persistent": [False, True]},
     "single_n24": {"total_units": [1024, 2048, 4096, 8192], "persistent": [False, True]},
     "uniform_n12_20": {"total_units": [2048, 4096, 8192], "persistent": [False, True]},
@@ -82,7 +82,7 @@ GRIDS = {
 # log2 of the mean Gray terms per item of each tuned workload class (the
 # tuner writes the same figure into the table as "log2_terms"; these are the
 # fallback for tables written before it did)
-CLASS_LOG2_TERMS = {"single_n32": 31.0, "single_n24": 23.0, "batched_uneven": 14.94,
+CLASS_LOG2_TERMS = {"single_n32": 31.0, "single_n24": 23.0, "batched_uneven": 15.16,
                     "uniform_n12_20": 16.84}
 SINGLE_CLASSES = ("single_n24", "single_n32")
 BATCH_CLASSES = ("batched_uneven", "uniform_n12_20")
