@@ -246,18 +246,12 @@ __device__ __forceinline__ void sweep_unit(const LaunchArgs &a, int item, int lo
   }
   double abs_sum = abs_acc * __builtin_fabs(lane_w);
   if constexpr (COMP) {
-    lane_sum = wave_reduce(lane_sum);
-    if constexpr (ABS) {
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) abs_sum += __shfl_xor(abs_sum, off);
-    }
+    lane_sum = wave_reduce<0>(lane_sum, lane);
+    if constexpr (ABS) abs_sum = wave_sum(abs_sum, lane);
   } else {
     // plain FP64 butterfly
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      lane_sum.re.hi += __shfl_xor(lane_sum.re.hi, off);
-      lane_sum.im.hi += __shfl_xor(lane_sum.im.hi, off);
-    }
+    lane_sum.re.hi = wave_sum(lane_sum.re.hi, lane);
+    lane_sum.im.hi = wave_sum(lane_sum.im.hi, lane);
   }
   if (total) {
     total->s = lane_sum;
@@ -372,15 +366,11 @@ __device__ __forceinline__ void finalize_item(const LaunchArgs &a, int item) {
     }
   }
   if constexpr (COMP) {
-    s = wave_reduce(s);
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) abs_sum += __shfl_xor(abs_sum, off);
+    s = wave_reduce<0>(s, lane);
+    abs_sum = wave_sum(abs_sum, lane);
   } else {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      s.re.hi += __shfl_xor(s.re.hi, off);
-      s.im.hi += __shfl_xor(s.im.hi, off);
-    }
+    s.re.hi = wave_sum(s.re.hi, lane);
+    s.im.hi = wave_sum(s.im.hi, lane);
   }
   if constexpr (NW > 1) {
     if (lane == 0) {
@@ -488,8 +478,9 @@ __global__ __launch_bounds__(kSingleBlockThreads) void perm_single_block_kernel(
   const int units = uniform(it->units);
   const int mcp = uniform(it->mcp);
   if (units <= W) {
-    // one unit per wave: the waves' totals meet in LDS and thread 0 adds
-    // them in unit order (a one-unit item has written its result itself)
+    // one unit per wave: the waves' totals meet in LDS and the first 16
+    // lanes of wave 0 add them with a fixed butterfly (a one-unit item has
+    // written its result itself)
     __shared__ double part[W][5];
     WaveTotal total{cdd_zero(), 0.0};
     if (wave < units) dispatch<LO, HI, R, true, true>(a, mcp, 0, wave, &total);
@@ -501,26 +492,33 @@ __global__ __launch_bounds__(kSingleBlockThreads) void perm_single_block_kernel(
       part[wave][4] = total.abs;
     }
     __syncthreads();
-    if (threadIdx.x == 0 && !(uniform(it->flags) & kDirectOut)) {
-      cdd s = cdd{{part[0][0], part[0][1]}, {part[0][2], part[0][3]}};
-      double abs_sum = part[0][4];
-      for (int w = 1; w < units; ++w) {
-        s = cdd_add(s, cdd{{part[w][0], part[w][1]}, {part[w][2], part[w][3]}});
-        abs_sum += part[w][4];
+    if (wave == 0 && !(uniform(it->flags) & kDirectOut)) {
+      // lane l < units holds unit l's total, the others zero; butterfly
+      // steps over lanes l ^ 1, l ^ 2, 7 - l and 15 - l leave the sum of
+      // lanes 0..15 (W = 16 waves) in every one of them, added in a fixed
+      // order; lane 0 writes the result
+      static_assert(W == 16, "the butterfly below covers 16 unit totals");
+      const int lane = static_cast<int>(threadIdx.x);
+      const bool has = lane < units;
+      cdd s = has ? cdd{{part[lane][0], part[lane][1]}, {part[lane][2], part[lane][3]}} : cdd_zero();
+      double abs_sum = has ? part[lane][4] : 0.0;
+      s = wave_reduce<0, 4>(s, lane);
+      abs_sum = wave_sum<0, 4>(abs_sum, lane);
+      if (lane == 0) {
+        const int e = -(it->n - 1);
+        const double re_hi = ldexp(s.re.hi, e), re_lo = ldexp(s.re.lo, e);
+        const double im_hi = ldexp(s.im.hi, e), im_lo = ldexp(s.im.lo, e);
+        const int o = it->out_index;
+        a.out[2 * o] = re_hi + re_lo;
+        a.out[2 * o + 1] = im_hi + im_lo;
+        if (a.out_dd) {
+          a.out_dd[4 * o] = re_hi;
+          a.out_dd[4 * o + 1] = re_lo;
+          a.out_dd[4 * o + 2] = im_hi;
+          a.out_dd[4 * o + 3] = im_lo;
+        }
+        if (a.out_abs) a.out_abs[o] = ldexp(abs_sum, e);
       }
-      const int e = -(it->n - 1);
-      const double re_hi = ldexp(s.re.hi, e), re_lo = ldexp(s.re.lo, e);
-      const double im_hi = ldexp(s.im.hi, e), im_lo = ldexp(s.im.lo, e);
-      const int o = it->out_index;
-      a.out[2 * o] = re_hi + re_lo;
-      a.out[2 * o + 1] = im_hi + im_lo;
-      if (a.out_dd) {
-        a.out_dd[4 * o] = re_hi;
-        a.out_dd[4 * o + 1] = re_lo;
-        a.out_dd[4 * o + 2] = im_hi;
-        a.out_dd[4 * o + 3] = im_lo;
-      }
-      if (a.out_abs) a.out_abs[o] = ldexp(abs_sum, e);
     }
   } else {
     for (int u = wave; u < units; u += W) dispatch<LO, HI, R, true, true>(a, mcp, 0, u);

@@ -83,18 +83,66 @@ __device__ __forceinline__ InnerEntry inner_entry(cptr<InnerEntry> tab, int v) {
   return e;
 }
 
-// Butterfly reduction of a complex dd value over the 64 lanes.
-__device__ __forceinline__ cdd wave_reduce(cdd v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    cdd o;
-    o.re.hi = __shfl_xor(v.re.hi, off);
-    o.re.lo = __shfl_xor(v.re.lo, off);
-    o.im.hi = __shfl_xor(v.im.hi, off);
-    o.im.lo = __shfl_xor(v.im.lo, off);
-    v = cdd_add(v, o);
+// The partner's value at step STEP of a butterfly over the 64 lanes: lanes
+// l ^ 1 and l ^ 2 (DPP quad permutations), then 7 - l and 15 - l within
+// groups of 8 and 16 (DPP half-row and row mirrors: after the previous steps
+// the two halves of each group hold the sums of disjoint lanes), then the
+// rows 16 and 32 lanes apart (v_permlane16_swap / v_permlane32_swap, which
+// exchange whole rows between a register pair; with the value in both
+// registers a lane of the lower row finds its partner's value in the second
+// result, a lane of the upper row in the first).  No LDS traffic: DPP and
+// the permlane swaps are VALU operations.
+template <int STEP>
+__device__ __forceinline__ double wave_partner(double x, int lane) {
+  const uint64_t v = __builtin_bit_cast(uint64_t, x);
+  uint32_t lo = static_cast<uint32_t>(v), hi = static_cast<uint32_t>(v >> 32);
+  if constexpr (STEP < 4) {
+    constexpr int kCtrl = STEP == 0 ? 0xB1 : STEP == 1 ? 0x4E : STEP == 2 ? 0x141 : 0x140;
+    lo = static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(lo), static_cast<int>(lo),
+                                                           kCtrl, 0xF, 0xF, false));
+    hi = static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(hi), static_cast<int>(hi),
+                                                           kCtrl, 0xF, 0xF, false));
+  } else {
+    const auto l = STEP == 4 ? __builtin_amdgcn_permlane16_swap(lo, lo, false, false)
+                             : __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto h = STEP == 4 ? __builtin_amdgcn_permlane16_swap(hi, hi, false, false)
+                             : __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    const bool upper = (lane & (STEP == 4 ? 16 : 32)) != 0;
+    lo = upper ? l[0] : l[1];
+    hi = upper ? h[0] : h[1];
   }
-  return v;
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+__device__ __forceinline__ int lane_id() {
+  return static_cast<int>(__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)));
+}
+
+// Butterfly reduction of a complex dd value over the 64 lanes (every lane
+// holds the sum; the additions are symmetric, so all lanes agree).  Every
+// lane of the wave must be active.
+// Steps STEP..END-1 only: END = 4 sums groups of 16 lanes.
+template <int STEP = 0, int END = 6>
+__device__ __forceinline__ cdd wave_reduce(cdd v, int lane) {
+  if constexpr (STEP < END) {
+    const cdd o{{wave_partner<STEP>(v.re.hi, lane), wave_partner<STEP>(v.re.lo, lane)},
+                {wave_partner<STEP>(v.im.hi, lane), wave_partner<STEP>(v.im.lo, lane)}};
+    return wave_reduce<STEP + 1, END>(cdd_add(v, o), lane);
+  } else {
+    return v;
+  }
+}
+__device__ __forceinline__ cdd wave_reduce(cdd v) { return wave_reduce<0, 6>(v, lane_id()); }
+
+// Plain FP64 sum over the 64 lanes (or groups of 16 for END = 4), same
+// butterfly.
+template <int STEP = 0, int END = 6>
+__device__ __forceinline__ double wave_sum(double x, int lane) {
+  if constexpr (STEP < END) {
+    return wave_sum<STEP + 1, END>(x + wave_partner<STEP>(x, lane), lane);
+  } else {
+    return x;
+  }
 }
 
 // Waves per SIMD the register allocator must leave room for: the row sums
