@@ -1420,8 +1420,11 @@ PlanOptions single_plan_options(uint32_t flags, const PermInput &in) {
   // from the sweep kernel
   opt.auto_units_cap = static_cast<int64_t>(std::max(dev.cus, 1)) * kSingleUnitsPerCu;
   const bool rows_ok = in.m_r == 0 || (in.m_r > 0 && in.rows);
-  if (!(flags & PQP_FLAG_PLAIN) && rows_ok && gray_terms(in) <= kSingleBlockMaxTerms)
-    opt.auto_units_cap = kSingleBlockUnits;
+  if (!(flags & PQP_FLAG_PLAIN) && rows_ok) {
+    const double t = gray_terms(in);
+    if (t <= kSingleBlockMaxTerms)
+      opt.auto_units_cap = t <= kSingleBlockSmallTerms ? kSingleBlockUnitsSmall : kSingleBlockUnits;
+  }
   opt.direct_out = (flags & PQP_FLAG_PLAIN) == 0;
   return opt;
 }

@@ -25,11 +25,15 @@ double gray_terms(const PermInput &in);
 
 // Single permanents of at most this many Gray terms and at most 16 columns
 // run in one launch of one block (perm_single_block_kernel), planned for at
-// most kSingleBlockUnits waves (8 measured faster than 16 at n = 12 and 14:
-// each lane's start state costs about as much as a few Gray steps, so fewer,
-// longer units do less work); larger ones spread over the device.
+// most kSingleBlockUnits waves, kSingleBlockUnitsSmall up to
+// kSingleBlockSmallTerms terms.  Each lane's start state costs about as much
+// as a few Gray steps and is latency-bound, so fewer, longer units finish
+// sooner (kernel time, MI355X: n = 12 at 4 / 8 / 16 waves 7.9 / 8.3 / 10.2
+// us; n = 14 14.2 / 13.6 / 15.4 us); larger ones spread over the device.
 constexpr double kSingleBlockMaxTerms = 16384.0;
 constexpr int kSingleBlockUnits = 8;
+constexpr double kSingleBlockSmallTerms = 2048.0;
+constexpr int kSingleBlockUnitsSmall = 4;
 
 // Planning options of a single-permanent call of `in` with these
 // PQP_FLAG_* flags: about 1.4 sqrt(T) waves, capped at 16 for the
