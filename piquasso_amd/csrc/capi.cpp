@@ -1374,12 +1374,55 @@ bool single_block_eligible(const Plan &plan, uint32_t flags, int *bucket) {
 void single_block_args(const Plan &plan, const LaunchArgs &a, uint64_t *done, uint64_t seq,
                        SingleArgs &args) {
   const ItemDesc &d = plan.items[0];
-  const size_t b_bytes = plan.info[0].rowsrc.size() * static_cast<size_t>(d.mcp) * 2 *
-                         (plan.fp32 ? sizeof(float) : sizeof(double));
+  const size_t rsz = plan.fp32 ? sizeof(float) : sizeof(double);
+  const size_t row_bytes = static_cast<size_t>(d.mcp) * 2 * rsz;
+  const size_t b_bytes = plan.info[0].rowsrc.size() * row_bytes;
   args.a = a;
   args.arena_lines = static_cast<int32_t>(plan.arena.size() / 64);  // whole lines only
   args.b_in_args = b_bytes <= static_cast<size_t>(kSingleArgBBytes) ? 1 : 0;
-  if (args.b_in_args) std::memcpy(args.b, plan.arena.data() + d.b_off, b_bytes);
+  args.base_in_args = 0;
+  args.base_off = 0;
+  // Binary top digits beyond the lane-dependent ones: each unit's start row
+  // without them, so that every lane adds kLaneDigits rows instead of all
+  // 1 + L + Dt (the start state is a chain of dependent row updates).  The
+  // kernel then reads rows 1 .. L + kLaneDigits of B only (the inner rows
+  // for the sweep, the lane digits' rows for the start), so the rows of the
+  // other top digits are left out of the arguments.
+  const int lane_rows = 1 + d.L + kLaneDigits;
+  const size_t base_off = (static_cast<size_t>(lane_rows) * row_bytes + 63) / 64 * 64;
+  const size_t base_bytes = static_cast<size_t>(d.units) * row_bytes;
+  if ((d.flags & kTopBinary) && d.Dt > kLaneDigits && d.units > 0 &&
+      d.units <= kSingleBlockThreads / kWave && base_off + base_bytes <= static_cast<size_t>(kSingleArgBBytes)) {
+    const int mcp = d.mcp, L = d.L;
+    const char *B = plan.arena.data() + d.b_off;
+    const double *alpha = reinterpret_cast<const double *>(plan.arena.data() + d.alpha_off);
+    auto row = [&](int r, int j, int c) -> double {  // component c of column j of row r of B
+      const size_t e = (static_cast<size_t>(r) * mcp + j) * 2 + c;
+      return plan.fp32 ? static_cast<double>(reinterpret_cast<const float *>(B)[e])
+                       : reinterpret_cast<const double *>(B)[e];
+    };
+    std::memcpy(args.b, B, static_cast<size_t>(lane_rows) * row_bytes);
+    for (int w = 0; w < d.units; ++w) {
+      const uint64_t u0 = static_cast<uint64_t>(w) * kWave;
+      const uint64_t g = u0 ^ (u0 >> 1);
+      for (int j = 0; j < mcp; ++j)
+        for (int c = 0; c < 2; ++c) {
+          double v = row(0, j, c);
+          for (int i = 1; i <= L; ++i) v += alpha[i - 1] * row(i, j, c);
+          for (int t = kLaneDigits; t < d.Dt; ++t) v += (((g >> t) & 1) ? -1.0 : 1.0) * row(L + 1 + t, j, c);
+          const size_t e = (static_cast<size_t>(w) * mcp + j) * 2 + c;
+          if (plan.fp32)
+            reinterpret_cast<float *>(args.b + base_off)[e] = static_cast<float>(v);
+          else
+            reinterpret_cast<double *>(args.b + base_off)[e] = v;
+        }
+    }
+    args.b_in_args = 1;
+    args.base_in_args = 1;
+    args.base_off = static_cast<int32_t>(base_off);
+  } else if (args.b_in_args) {
+    std::memcpy(args.b, plan.arena.data() + d.b_off, b_bytes);
+  }
   args.done = done;
   args.seq = seq;
 }

@@ -147,7 +147,7 @@ struct WaveTotal {
 // One wave unit: top states [64*local, 64*local+64) of item `item`.  ABS
 // (compensated static launches): also sum the magnitudes of the terms.
 // `total`, when given, receives the unit's sum as well.
-template <int MCP, typename R, bool COMP, bool ABS>
+template <int MCP, typename R, bool COMP, bool ABS, bool UNIT_BASE = false>
 __device__ __forceinline__ void sweep_unit(const LaunchArgs &a, int item, int local,
                                            WaveTotal *total = nullptr) {
   const cptr<ItemDesc> it = as_const<ItemDesc>(a.items) + item;
@@ -177,7 +177,7 @@ __device__ __forceinline__ void sweep_unit(const LaunchArgs &a, int item, int lo
 
   R cr[MCP], ci[MCP];
   double w_top;
-  start_state<MCP, R>(a, it, u, L, Dt, flags, cr, ci, w_top);
+  start_state<MCP, R, UNIT_BASE>(a, it, u, L, Dt, flags, cr, ci, w_top);
   // (-1)^{|g_top|} = (-1)^u: consecutive top states differ by one unit step.
   double lane_w = (u & 1) ? -w_top : w_top;
   if (!active) lane_w = 0.0;
@@ -284,14 +284,14 @@ __device__ __forceinline__ void sweep_unit(const LaunchArgs &a, int item, int lo
   }
 }
 
-template <int LO, int HI, typename R, bool COMP, bool ABS>
+template <int LO, int HI, typename R, bool COMP, bool ABS, bool UNIT_BASE = false>
 __device__ __forceinline__ void dispatch(const LaunchArgs &a, int mcp, int item, int local,
                                          WaveTotal *total = nullptr) {
   if constexpr (LO <= HI) {
     if (mcp == LO) {
-      sweep_unit<LO, R, COMP, ABS>(a, item, local, total);
+      sweep_unit<LO, R, COMP, ABS, UNIT_BASE>(a, item, local, total);
     } else {
-      dispatch<LO + 1, HI, R, COMP, ABS>(a, mcp, item, local, total);
+      dispatch<LO + 1, HI, R, COMP, ABS, UNIT_BASE>(a, mcp, item, local, total);
     }
   }
 }
@@ -441,7 +441,7 @@ __global__ __launch_bounds__(NT) void perm_finalize_kernel(LaunchArgs a, int ite
 // The rows of B may come in the kernel arguments (SingleArgs::b): the
 // argument struct is read through the kernel-argument segment pointer, as
 // scalar loads like the arena.
-template <int LO, int HI, typename R>
+template <int LO, int HI, typename R, bool UNIT_BASE>
 __global__ __launch_bounds__(kSingleBlockThreads) void perm_single_block_kernel(SingleArgs args) {
   constexpr int W = kSingleBlockThreads / kWave;
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
@@ -454,6 +454,7 @@ __global__ __launch_bounds__(kSingleBlockThreads) void perm_single_block_kernel(
     a.b_rows = reinterpret_cast<const char *>(b_addr);
     b_lines = kSingleArgBBytes / 64;
   }
+  if (UNIT_BASE) a.unit_base = reinterpret_cast<const char *>(b_addr + args.base_off);
   {
     const cptr<uint32_t> lines = as_const<uint32_t>(a.arena);
     const cptr<uint32_t> blines = reinterpret_cast<cptr<uint32_t>>(b_addr);
@@ -483,7 +484,7 @@ __global__ __launch_bounds__(kSingleBlockThreads) void perm_single_block_kernel(
     // written its result itself)
     __shared__ double part[W][5];
     WaveTotal total{cdd_zero(), 0.0};
-    if (wave < units) dispatch<LO, HI, R, true, true>(a, mcp, 0, wave, &total);
+    if (wave < units) dispatch<LO, HI, R, true, true, UNIT_BASE>(a, mcp, 0, wave, &total);
     if ((threadIdx.x & (kWave - 1)) == 0 && wave < units) {
       part[wave][0] = total.s.re.hi;
       part[wave][1] = total.s.re.lo;
@@ -619,8 +620,13 @@ hipError_t PQP_NAME(occupancy_fwd_b)(bool persistent, int *blocks) {
 
 #if PQP_BUCKET_ID <= 1 && PQP_VARIANT != 1
 hipError_t PQP_NAME(launch_single_block_b)(const SingleArgs &args, hipStream_t stream) {
-  hipLaunchKernelGGL((perm_single_block_kernel<PQP_LO, PQP_HI, PQP_R>), dim3(1),
-                     dim3(kSingleBlockThreads), 0, stream, args);
+  // the instantiation with per-unit start rows only where the host made them
+  if (args.base_in_args)
+    hipLaunchKernelGGL((perm_single_block_kernel<PQP_LO, PQP_HI, PQP_R, true>), dim3(1),
+                       dim3(kSingleBlockThreads), 0, stream, args);
+  else
+    hipLaunchKernelGGL((perm_single_block_kernel<PQP_LO, PQP_HI, PQP_R, false>), dim3(1),
+                       dim3(kSingleBlockThreads), 0, stream, args);
   return hipGetLastError();
 }
 #endif

@@ -129,7 +129,17 @@ struct LaunchArgs {
   // arena + b_off (the copy a one-block launch carries in its kernel
   // arguments); null everywhere else.
   const char *b_rows = nullptr;
+  // One-block launches of items whose top digits are all binary: per unit,
+  // the row sums of its first state less the lowest kLaneDigits top digits
+  // (the prepended row, the inner rows at their start coefficients and the
+  // top digits that are the same for all 64 lanes of the unit), m_cp values
+  // of the item's real type per unit, computed on the host; null elsewhere.
+  const char *unit_base = nullptr;
 };
+
+// Top digits that differ between the 64 lanes of a unit (lane l of unit w
+// holds top state 64 w + l): the reflected binary digits 0..5.
+constexpr int kLaneDigits = 6;
 
 // Kernel arguments of the one-block single-permanent launch: the launch
 // arguments, the done slot it publishes to, and up to kSingleArgBBytes of
@@ -141,6 +151,8 @@ struct SingleArgs {
   LaunchArgs a;
   int32_t arena_lines;  // 64-byte lines of the arena the block touches first
   int32_t b_in_args;    // 1: rows of B in b[] (a.b_rows is set in the kernel)
+  int32_t base_in_args; // 1: per-unit start rows in b[] at base_off (a.unit_base is set in the kernel)
+  int32_t base_off;     // byte offset of those rows in b[]
   uint64_t *done;       // host-visible slot receiving seq
   uint64_t seq;
   alignas(64) char b[kSingleArgBBytes];

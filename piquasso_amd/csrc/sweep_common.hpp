@@ -162,7 +162,7 @@ __device__ __forceinline__ cptr<R> b_rows(const LaunchArgs &a, const cptr<ItemDe
   return a.b_rows ? as_const<R>(a.b_rows) : as_const<R>(a.arena, it->b_off);
 }
 
-template <int MCP, typename R>
+template <int MCP, typename R, bool UNIT_BASE = false>
 __device__ __forceinline__ void start_state(const LaunchArgs &a, const cptr<ItemDesc> it,
                                             int64_t u, int L, int Dt, int flags,
                                             R (&cr)[MCP], R (&ci)[MCP], double &w_top) {
@@ -182,13 +182,34 @@ __device__ __forceinline__ void start_state(const LaunchArgs &a, const cptr<Item
   // floor(u / P) is evaluated as floor((u + 1/2) * (1/P)) in FP64, exact for
   // u < 2^50 (the fraction stays >= 1/(2P) away from an integer).  For
   // all-binary top digits this is the reflected binary code u ^ (u >> 1).
+  const uint64_t gbin = static_cast<uint64_t>(u) ^ (static_cast<uint64_t>(u) >> 1);
+  if (UNIT_BASE && a.unit_base) {
+    // one-block launch, binary top digits: the unit's start row (host sum of
+    // the rows common to its 64 lanes) plus the lane-dependent digits
+    const cptr<R> base = as_const<R>(a.unit_base) + (u / kWave) * 2 * MCP;
+#pragma unroll
+    for (int j = 0; j < MCP; ++j) {
+      cr[j] = base[2 * j];
+      ci[j] = base[2 * j + 1];
+    }
+    w_top = 1.0;
+    for (int t = (Dt < kLaneDigits ? Dt : kLaneDigits) - 1; t >= 0; --t) {
+      const R al = ((gbin >> t) & 1) ? R(-1) : R(1);
+      const cptr<R> row = B + static_cast<int64_t>(L + 1 + t) * 2 * MCP;
+#pragma unroll
+      for (int j = 0; j < MCP; ++j) {
+        cr[j] = fmaR(al, row[2 * j], cr[j]);
+        ci[j] = fmaR(al, row[2 * j + 1], ci[j]);
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < MCP; ++j) {
     cr[j] = R(0);
     ci[j] = R(0);
   }
   const cptr<double> top = as_const<double>(a.arena, it->top_off);
-  const uint64_t gbin = static_cast<uint64_t>(u) ^ (static_cast<uint64_t>(u) >> 1);
   const double ud = static_cast<double>(u) + 0.5;
   double q_hi = 0.0;
   int parity = 0;
