@@ -877,6 +877,29 @@ int run_single(const PermInput &in, uint32_t flags, double *out2, double *out_dd
     x.abs_valid = true;
     PQP_HIP(launch_single_block(bucket, fp32, args, x.stream));
     if ((st = wait_done(x, done, args.seq)) != PQP_OK) return st;
+  } else if (!(flags & PQP_FLAG_PLAIN) && x.plan.ranges.size() == 1 && d.units > 0 &&
+             d.mcp <= kLastFinalizeMaxWidth) {
+    // one static launch over the device whose last block finalizes the
+    // permanent and publishes the result (no finalize launch, no stream
+    // synchronisation)
+    volatile uint64_t *done = reinterpret_cast<volatile uint64_t *>(res + 7 * x.n_results);
+    FwdLaunch l;
+    l.args = x.args();
+    l.args.last_counter = static_cast<int *>(x.heads.p) + kLastCounterSlot;
+    l.args.done = reinterpret_cast<uint64_t *>(static_cast<double *>(x.results_m.dev) + 7 * x.n_results);
+    l.args.seq = ++S.xs_seq;
+    l.first_unit = 0;
+    l.n_units = d.units;
+    l.persistent = false;
+    l.fp32 = fp32;
+    l.comp = true;
+    l.stream = x.stream;
+    int mcp = 0;
+    const int b = bucket_for_width(d.mcp, fp32, &mcp);
+    *done = 0;
+    x.abs_valid = true;
+    PQP_HIP(launch_fwd(b, l));
+    if ((st = wait_done(x, done, l.args.seq)) != PQP_OK) return st;
   } else {
     if ((st = enqueue(x, (flags & PQP_FLAG_PLAIN) ? PQP_RUN_PLAIN : PQP_RUN_EAGER, nullptr,
                       nullptr)) != PQP_OK)

@@ -321,16 +321,6 @@ __attribute__((amdgpu_waves_per_eu(waves_for(HI, sizeof(R) == 4), 8))) void perm
   }
 }
 
-// Static launch: block b sweeps unit first_unit + b.
-template <int LO, int HI, typename R, bool COMP>
-__global__ __launch_bounds__(kWave)
-__attribute__((amdgpu_waves_per_eu(waves_for(HI, sizeof(R) == 4), 8))) void perm_static_kernel(LaunchArgs a, int first_unit) {
-  const cptr<WorkUnit> wu = as_const<WorkUnit>(a.units) + (first_unit + static_cast<int>(blockIdx.x));
-  const int item = uniform(wu->item);
-  const int local = uniform(wu->local);
-  const int mcp = uniform(as_const<ItemDesc>(a.items)[item].mcp);
-  dispatch<LO, HI, R, COMP, COMP>(a, mcp, item, local);
-}
 
 // The block's sum of one item's partials: thread t adds partials t, t + NT,
 // ... (double-double), each wave reduces with a butterfly, and the
@@ -416,6 +406,43 @@ __device__ __forceinline__ void finalize_item(const LaunchArgs &a, int item) {
 // One block of NT threads per item (finalize_item).  NT = 64 for batches of
 // small items, 1024 when an item has thousands of units.  Block 0 also
 // rewinds the work-queue heads for the next launch of the same plan.
+// Static launch: block b sweeps unit first_unit + b.  A one-item launch of a
+// single permanent (a.last_counter set) also finalizes the item in the block
+// that finishes last: every block makes its partial visible device-wide,
+// counts itself, and the block that counts last reads all partials (the
+// fences order the partial writes, on any XCD, before the count and the
+// count before the reads), writes the result to mapped host memory, resets
+// the counter for the next launch on the stream and publishes the sequence
+// number, so the caller polls instead of waiting for a finalize launch.
+template <int LO, int HI, typename R, bool COMP>
+__global__ __launch_bounds__(kWave)
+__attribute__((amdgpu_waves_per_eu(waves_for(HI, sizeof(R) == 4), 8))) void perm_static_kernel(LaunchArgs a, int first_unit) {
+  const cptr<WorkUnit> wu = as_const<WorkUnit>(a.units) + (first_unit + static_cast<int>(blockIdx.x));
+  const int item = uniform(wu->item);
+  const int local = uniform(wu->local);
+  const int mcp = uniform(as_const<ItemDesc>(a.items)[item].mcp);
+  dispatch<LO, HI, R, COMP, COMP>(a, mcp, item, local);
+  // widths up to 16 only (kLastFinalizeMaxWidth): in the wider kernels the
+  // finalize code took private memory
+  if constexpr (COMP && HI <= kLastFinalizeMaxWidth) {
+    if (a.last_counter) {
+      __threadfence();
+      int old = 0;
+      if (threadIdx.x == 0) old = atomicAdd(a.last_counter, 1);
+      old = __builtin_amdgcn_readfirstlane(old);
+      if (old == static_cast<int>(gridDim.x) - 1) {
+        __threadfence();
+        finalize_item<true, kWave>(a, item);
+        if (threadIdx.x == 0) {
+          *a.last_counter = 0;
+          __threadfence_system();
+          __hip_atomic_store(a.done, a.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+      }
+    }
+  }
+}
+
 template <bool COMP, int NT>
 __global__ __launch_bounds__(NT) void perm_finalize_kernel(LaunchArgs a, int item0, int n_heads) {
   const int item = item0 + static_cast<int>(blockIdx.x);

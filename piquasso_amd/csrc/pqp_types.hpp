@@ -135,7 +135,19 @@ struct LaunchArgs {
   // top digits that are the same for all 64 lanes of the unit), m_cp values
   // of the item's real type per unit, computed on the host; null elsewhere.
   const char *unit_base = nullptr;
+  // One-item static launches of a single permanent: each block counts itself
+  // at last_counter after writing its partial; the block that counts last
+  // sums the item's partials, writes the result, resets the counter and
+  // publishes seq at done (mapped host memory).  Null elsewhere.
+  int *last_counter = nullptr;
+  uint64_t *done = nullptr;
+  uint64_t seq = 0;
 };
+
+// The queue-head slot that holds a single permanent's block counter, and
+// the widest kernel that finalizes in its last block.
+constexpr int kLastCounterSlot = 63;
+constexpr int kLastFinalizeMaxWidth = 16;
 
 // Top digits that differ between the 64 lanes of a unit (lane l of unit w
 // holds top state 64 w + l): the reflected binary digits 0..5.

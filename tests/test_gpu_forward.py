@@ -360,3 +360,32 @@ def test_superposition_particle_number_probabilities(gpu, oracle):
     assert abs(get_ideal_particle_number_probability(outs[7], u, ins, c) - p[7]) < 1e-15
     amp = calculate_inner_product(u, ins[0], outs[40])
     assert abs(amp - complex(pa.output_amplitudes(u, ins[0], [outs[40]])[0])) == 0
+
+
+def test_single_calls_around_the_one_block_limit(gpu, oracle):
+    """Single permanents just below and above 16 384 Gray terms: one block of
+    the device (per-unit start rows where the top digits are binary) or one
+    static launch over the device whose last block finalizes the permanent
+    (its block counter must be back at zero for the next call).  Calls of
+    each kind follow one another, every result against the double-double
+    oracle."""
+    rng = np.random.default_rng(12)
+    cases = [
+        (np.ones(12, int), np.ones(12, int)),            # 2^11 terms, binary top digits
+        (np.ones(13, int), np.ones(13, int)),            # 2^12
+        (np.ones(16, int), np.ones(16, int)),            # 2^15: spread, width 16
+        (np.array([1] * 11 + [2] * 3), np.array([2] + [1] * 15)),  # 27 648 terms: spread, mixed radix
+        (np.ones(15, int), np.ones(15, int)),            # 2^14: one block at the limit
+    ]
+    for rep in range(3):
+        for r, c in cases:
+            a = (rng.standard_normal((len(r), len(c))) + 1j * rng.standard_normal((len(r), len(c)))) / 3
+            got = complex(pa.permanent(a, r, c))
+            ref = oracle.perm_dd(a, r, c)
+            assert rel_err(got, ref) < TOL, (rep, r.tolist(), c.tolist())
+    # complex64 input takes the FP32-product variant of the same launches
+    a = (rng.standard_normal((16, 16)) + 1j * rng.standard_normal((16, 16))).astype(np.complex64) / 3
+    ones = np.ones(16, int)
+    ref = oracle.perm_dd(a.astype(np.complex128), ones, ones)
+    got = complex(pa.permanent(a, ones, ones))
+    assert rel_err(got, ref) < 1e-3
