@@ -336,8 +336,18 @@ def _gradient_leg(pa, W, reps: int, cpu=None):
         gb.sync()
         perms = gb.results()
         grads = gb.gradients()
-        ms = gb.time("graph", reps=reps)
+        # both launch modes of the same sweeps, timed alike after untimed
+        # evaluations of their own (the clock rises under sustained load); the
+        # faster is quoted
+        ms_by = {}
+        for m in ("graph", "eager"):
+            for _ in range(max(3, reps)):
+                gb.run(m)
+            gb.sync()
+            ms_by[m] = gb.time(m, reps=reps)
         st = gb.stats()
+    launch = min(ms_by, key=lambda m: float(np.median(ms_by[m])))
+    ms = ms_by[launch]
     resid = 0.0
     for a, r, p, g in zip(mats, rows, perms, grads):
         n = int(np.sum(r))
@@ -358,8 +368,11 @@ def _gradient_leg(pa, W, reps: int, cpu=None):
         "euler_residual_max": float(resid),
         "fp64_peak_frac": st["flops"] / (med * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
         "flop_model": "sum over items of T * (8 m_c + 18 (n - 1) + 4), T = Gray terms",
+        "launch": launch,
+        "ms_by_launch": {m: float(np.median(v)) for m, v in ms_by.items()},
         "what": "256 gradients d perm/dA (n~U{12..20}, 48-mode Haar submatrices): one Gray sweep "
-                "per permanent for the permanent and all n^2 gradient entries, one hipGraph replay",
+                "per permanent for the permanent and all n^2 gradient entries; hipGraph replay and "
+                "direct launches timed alike, the faster quoted",
     }
 
 
@@ -532,8 +545,10 @@ def _sampling_leg(pa, W, cpu):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # 100 evaluations of the batch take ~40 ms; 20 untimed ones before each
+    # launch mode bring the device to its sustained clock first
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-child", type=float, default=None)
@@ -620,8 +635,13 @@ def main():
             _, cus, arch = _capi.device_info()
             cfg32 = best_config(1, cus, arch, log2_terms=31.0) or {"persistent": False, "total_units": 128 * cus}
             b = pa.PermanentBatch([a32], [r32], [c32], **cfg32)
+        # --warmup untimed evaluations first (traced too: the summaries
+        # average the last --steps dispatches), at the sustained clock
+        for _ in range(args.warmup):
+            b.run(args.replay)
+        b.sync()
         ms = b.time(args.replay, reps=args.steps)
-        print(json.dumps({"profile": args.profile, "ms": ms.tolist()}))
+        print(json.dumps({"profile": args.profile, "warmup": args.warmup, "ms": ms.tolist()}))
         return 0
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
