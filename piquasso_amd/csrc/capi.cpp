@@ -899,7 +899,13 @@ int run_single(const PermInput &in, uint32_t flags, double *out2, double *out_dd
     *done = 0;
     x.abs_valid = true;
     PQP_HIP(launch_fwd(b, l));
-    if ((st = wait_done(x, done, l.args.seq)) != PQP_OK) return st;
+    if ((st = wait_done(x, done, l.args.seq)) != PQP_OK) {
+      // a launch that did not finish its count leaves it behind: clear it so
+      // that the next call's last block is recognised
+      (void)hipMemsetAsync(l.args.last_counter, 0, sizeof(int), x.stream);
+      (void)hipStreamSynchronize(x.stream);
+      return st;
+    }
   } else {
     if ((st = enqueue(x, (flags & PQP_FLAG_PLAIN) ? PQP_RUN_PLAIN : PQP_RUN_EAGER, nullptr,
                       nullptr)) != PQP_OK)
