@@ -187,7 +187,8 @@ int pqp_permanent_int64(const int64_t *matrix, int64_t m_r, int64_t m_c, int64_t
  *   rows, cols   : count pointers to int32 multiplicities
  *   target_inner : inner Gray states per lane (0 = the planner's cost model)
  *   total_units  : wave work units to spread over the batch in proportion to
- *                  each permanent's cost (0 = 48 per compute unit)
+ *                  each permanent's cost (0 = 32 per compute unit; gradient
+ *                  batches 8 per compute unit)
  */
 typedef struct pqp_batch pqp_batch;
 
