@@ -42,6 +42,20 @@ def test_ctypes_table_covers_the_header():
     assert set(declared_functions()) == set(_capi.SIGNATURES)
 
 
+def test_constants_match_the_header():
+    """Every PQP_* constant of the header (status codes, flags, run modes) has
+    the same value in the Python binding."""
+    text = open(HEADER).read()
+    defs = dict(re.findall(r"#define\s+(PQP_[A-Z0-9_]+)\s+\(?(-?(?:0x)?[0-9a-fA-F]+)u?\)?", text))
+    assert len(defs) >= 15
+    for name, value in defs.items():
+        if hasattr(_capi, name):
+            assert getattr(_capi, name) == int(value, 0), name
+    for name in ("PQP_FLAG_AS_GIVEN", "PQP_FLAG_GRADIENT", "PQP_PLAN_GRADIENT_SWEEP", "PQP_RUN_KERNEL",
+                 "PQP_ERR_NOKERNEL"):
+        assert name in defs and hasattr(_capi, name), name
+
+
 def test_version_and_device_query_without_gpu():
     L = _capi.lib()
     assert L.pqp_version().decode().startswith("pqp-permanent")
