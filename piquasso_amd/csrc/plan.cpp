@@ -18,6 +18,19 @@ constexpr int kExpandAutoMax = 0;                 // auto mode: never (measured 
 
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
+// Stable insertion sort for the planner's short per-item lists (at most 64
+// columns or digits): std::stable_sort takes a temporary buffer from the
+// heap on every call, which dominated planning batches of small items.
+template <typename T, typename Less>
+void small_stable_sort(std::vector<T> &v, Less less) {
+  for (size_t i = 1; i < v.size(); ++i) {
+    T x = v[i];
+    size_t j = i;
+    for (; j > 0 && less(x, v[j - 1]); --j) v[j] = v[j - 1];
+    v[j] = x;
+  }
+}
+
 int64_t binom64(int n, int k) {
   if (k < 0 || k > n) return 0;
   if (k > n - k) k = n - k;
@@ -329,9 +342,7 @@ int plan_build(const std::vector<PermInput> &caller_inputs, const PlanOptions &o
     colsel.clear();
     for (int64_t j = 0; j < in.m_c; ++j)
       if (in.cols[j] > 0) colsel.push_back(j);
-    std::stable_sort(colsel.begin(), colsel.end(), [&](int64_t x, int64_t y) {
-      return in.cols[x] > in.cols[y];
-    });
+    small_stable_sort(colsel, [&](int64_t x, int64_t y) { return in.cols[x] > in.cols[y]; });
     const int mc_alg = static_cast<int>(colsel.size());  // algorithmic column count
     int cmax_in = 0;
     for (int64_t j : colsel) cmax_in = std::max(cmax_in, static_cast<int>(in.cols[j]));
@@ -413,9 +424,7 @@ int plan_build(const std::vector<PermInput> &caller_inputs, const PlanOptions &o
     // --- inner/top split: inner digits = smallest radices first ---
     order.resize(D);
     std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-      return digits[x].rho < digits[y].rho;
-    });
+    small_stable_sort(order, [&](int x, int y) { return digits[x].rho < digits[y].rho; });
     int cmax = 0;
     for (int c : colmult) cmax = std::max(cmax, c);
     const double step_cost =
