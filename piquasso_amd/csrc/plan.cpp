@@ -668,8 +668,27 @@ int plan_build(const std::vector<PermInput> &caller_inputs, const PlanOptions &o
                                                 4.0 * (mc_alg + 1))
                 : opt.grad ? static_cast<double>(T) * (8.0 * mc_alg + 18.0 * (n - 1) + 4.0)
                            : static_cast<double>(T) * (2.0 * mc_alg + 6.0 * (n - 1) + 4.0);
-    inf.unit_cost = ((1.0 + D) * 2.0 * mcp + Dt * 40.0 + 400.0) +
-                    static_cast<double>(S_in) * step_cost;
+    if (opt.minors || opt.grad) {
+      inf.unit_cost = ((1.0 + D) * 2.0 * mcp + Dt * 40.0 + 400.0) +
+                      static_cast<double>(S_in) * step_cost;
+    } else {
+      // LPT key of a forward unit: the VALU instructions its wave issues.
+      // Per state the row update (2 mcp), the product chain (4 (mcp - 1)),
+      // the column powers (grouped: 4 (cmax - 1) multiplies, plus one scalar
+      // bit test per column at half weight; counts: 4 (n - m_c)) and the
+      // accumulation (8); per unit the start state (2 mcp per row of B, the
+      // top-digit decode) and the wave reduction.  The choice of the split
+      // above keeps the coarser model; this one orders the units, where an
+      // overestimate of the grouped items' products (about 4 (n - m_c) for
+      // 4 (cmax - 1)) would run long units last and lengthen the launch's
+      // tail.
+      const double powers = cmax == 1 ? 0.0
+                            : grouped ? 4.0 * (cmax - 1) + 0.5 * mcp
+                                      : 4.0 * (n - mc);
+      const double state = 2.0 * mcp + 4.0 * (mcp - 1) + powers + 8.0;
+      inf.unit_cost = (1.0 + D) * 2.0 * mcp + Dt * (top_binary ? 4.0 : 60.0) + 300.0 +
+                      static_cast<double>(S_in) * state;
+    }
     plan.total_flops += inf.flops;
     plan.total_terms += T;
     item_units[k] = d.units;
