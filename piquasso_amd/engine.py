@@ -116,8 +116,10 @@ class PermanentBatch:
         None (default): the autotuned choice for this device
         (:func:`piquasso_amd.tune.best_config`), else False.
     target_inner:
-        inner Gray states per lane (0 lets the planner's cost model choose;
-        the autotuner in :mod:`piquasso_amd.tune` picks it per workload).
+        inner Gray states per lane (at most; the largest Gray sub-space of
+        the item that fits).  0: the autotuned value for this device when
+        total_units is not given either (:mod:`piquasso_amd.tune`), else
+        the planner's cost model chooses per item.
     expand_cols:
         None (planner's choice, currently keep), True (columns of multiplicity
         c > 1 become c identical columns) or False (keep one column whose row
@@ -171,6 +173,10 @@ class PermanentBatch:
                 persistent = bool(tuned.get("persistent", False))
             if not total_units:
                 total_units = int(tuned.get("total_units", 0))
+                # a tuned inner-sweep length replaces the split total_units
+                # asks for, so it applies only where the caller gave neither
+                if not target_inner:
+                    target_inner = int(tuned.get("target_inner", 0))
         self.gradient = bool(gradient)
         n = self.count
         shared = _shared_marshal(matrices, rows, cols)

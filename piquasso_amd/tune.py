@@ -1,12 +1,15 @@
 """Autotuner for the permanent engine's launch parameters.
 
 The planner splits each permanent into wave work units (64 top Gray states
-per wave, an inner Gray sweep per lane).  How finely to split, whether to
-expand multiplicity columns, and whether to use the persistent work-stealing
-launch or one block per unit are workload dependent.  This module measures
-candidate settings on the device (HIP-event timing of the sweep kernels) and
-writes the fastest per workload class to a JSON table that
-:func:`load_tuning` reads back.
+per wave, an inner Gray sweep per lane).  How many units a batch gets
+(total_units, the waves per permanent in proportion to its cost), how long
+each lane's inner sweep is (target_inner, the chunk of Gray states per lane;
+0 leaves the split to the planner's cost model) and whether to use the
+persistent work-stealing launch or one block per unit are workload
+dependent.  This module measures candidate settings on the device (HIP-event
+timing of the sweep kernels, after untimed evaluations that bring the clock
+to its sustained value, in interleaved passes) and writes the fastest per
+workload class to a JSON table that :func:`load_tuning` reads back.
 
 CLI:  python -m piquasso_amd.tune [--out piquasso_amd/tuning/gfx950.json] [--reps 5]
 """
@@ -25,15 +28,22 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_TABLE = os.path.join(HERE, "tuning", "gfx950.json")
 
 
-def measure(mats, rows, cols, reps: int = 5, mode: str = "kernel", **cfg) -> dict:
-    """Median time of the batch under one configuration (PermanentBatch kwargs)."""
+def measure(mats, rows, cols, reps: int = 5, mode: str = "kernel", warm_s: float = 0.1, **cfg) -> dict:
+    """Median time of the batch under one configuration (PermanentBatch
+    kwargs), after about `warm_s` seconds (at least 3 evaluations) of
+    untimed evaluations: planning leaves the device idle, and its clock
+    drops meanwhile."""
     from .engine import PermanentBatch
 
     t0 = time.perf_counter()
     with PermanentBatch(mats, rows, cols, **cfg) as b:
         plan_s = time.perf_counter() - t0
-        b.run("eager")
-        b.sync()
+        t1 = time.perf_counter()
+        k = 0
+        while k < 3 or time.perf_counter() - t1 < warm_s:
+            b.run("eager")
+            b.sync()
+            k += 1
         ms = b.time(mode, reps=reps)
         st = b.stats()
     return {"config": cfg, "ms": float(np.median(ms)), "ms_all": ms.tolist(),
@@ -41,17 +51,37 @@ def measure(mats, rows, cols, reps: int = 5, mode: str = "kernel", **cfg) -> dic
             "tflops": st["flops"] / (float(np.median(ms)) * 1e-3) / 1e12}
 
 
-def sweep(mats, rows, cols, grid: dict, reps: int = 5, mode: str = "kernel", log=print):
+def sweep(mats, rows, cols, grid: dict, reps: int = 5, mode: str = "kernel", log=print,
+          passes: int = 2, warm_s: float = 0.3):
     """Measure every combination of `grid` (dict of PermanentBatch kwarg ->
-    candidate values); returns the records sorted fastest first."""
+    candidate values) in `passes` interleaved passes, after about `warm_s`
+    seconds of untimed evaluations of the first combination (the device's
+    clock rises under sustained load, and the first combinations would
+    otherwise run at a lower one); each record's time is the median over
+    the passes.  Returns the records sorted fastest first."""
+    from .engine import PermanentBatch
+
     keys = sorted(grid)
-    recs = []
-    for values in itertools.product(*[grid[k] for k in keys]):
-        cfg = dict(zip(keys, values))
-        rec = measure(mats, rows, cols, reps=reps, mode=mode, **cfg)
-        log(json.dumps({"config": cfg, "ms": round(rec["ms"], 4), "tflops": round(rec["tflops"], 2),
-                        "units": rec["units"]}))
-        recs.append(rec)
+    cfgs = [dict(zip(keys, values)) for values in itertools.product(*[grid[k] for k in keys])]
+    with PermanentBatch(mats, rows, cols, **cfgs[0]) as b:
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < warm_s:
+            b.run("eager")
+            b.sync()
+    recs = [None] * len(cfgs)
+    for p in range(max(passes, 1)):
+        for i, cfg in enumerate(cfgs):
+            rec = measure(mats, rows, cols, reps=reps, mode=mode, **cfg)
+            log(json.dumps({"pass": p, "config": cfg, "ms": round(rec["ms"], 4),
+                            "tflops": round(rec["tflops"], 2), "units": rec["units"]}))
+            if recs[i] is None:
+                rec["ms_passes"] = [rec["ms"]]
+                recs[i] = rec
+            else:
+                recs[i]["ms_passes"].append(rec["ms"])
+    for rec in recs:
+        rec["ms"] = float(np.median(rec["ms_passes"]))
+        rec["tflops"] = rec["flops"] / (rec["ms"] * 1e-3) / 1e12
     recs.sort(key=lambda r: r["ms"])
     return recs
 
@@ -71,11 +101,18 @@ def _workloads():
     }
 
 
+# target_inner > 0 sets every item's inner sweep to the largest Gray
+# sub-space of at most that many states and overrides the split total_units
+# asks for; 0 lets total_units and the cost model choose it per item.
 GRIDS = {
-    "batched_uneven": {"total_units": [4096, 5120, 6144, 8192, 12288], "persistent": [False, True]},
-    "single_n32": {"total_units": [16384, 32768, 65536], "persistent": [False, True]},
-    "single_n24": {"total_units": [1024, 2048, 4096, 8192], "persistent": [False, True]},
-    "uniform_n12_20": {"total_units": [2048, 4096, 8192], "persistent": [False, True]},
+    "batched_uneven": {"total_units": [4096, 5120, 6144, 8192, 12288], "persistent": [False, True],
+                       "target_inner": [0, 64, 256, 1024]},
+    "single_n32": {"total_units": [16384, 32768, 65536], "persistent": [False, True],
+                   "target_inner": [0, 16384]},
+    "single_n24": {"total_units": [1024, 2048, 4096, 8192], "persistent": [False, True],
+                   "target_inner": [0, 1024]},
+    "uniform_n12_20": {"total_units": [2048, 4096, 8192], "persistent": [False, True],
+                       "target_inner": [0, 128, 512]},
 }
 
 
@@ -110,7 +147,7 @@ def choose_class(table: dict, count: int, log2_terms=None) -> str:
 def best_config(count: int, cus: int, arch: str, path: str = None, log2_terms=None) -> dict:
     """Launch settings for a batch of `count` permanents of mean Gray term
     count 2^log2_terms from the tuned table of this device (PermanentBatch
-    kwargs: total_units, persistent), or {} when no table was written for
+    kwargs: total_units, persistent, target_inner), or {} when no table was written for
     it.  The entry is the tuned class of nearest size (choose_class); unit
     counts scale with the CU count of the table's device."""
     table = load_tuning(path or DEFAULT_TABLE)
@@ -123,7 +160,7 @@ def best_config(count: int, cus: int, arch: str, path: str = None, log2_terms=No
     cfg = dict(rec["best"])
     if "total_units" in cfg and table.get("cus"):
         cfg["total_units"] = int(round(cfg["total_units"] / table["cus"] * cus))
-    return {k: v for k, v in cfg.items() if k in ("total_units", "persistent")}
+    return {k: v for k, v in cfg.items() if k in ("total_units", "persistent", "target_inner")}
 
 
 def mean_log2_terms(rows, sample: int = 256) -> float:
@@ -153,7 +190,8 @@ def tune(out: str = DEFAULT_TABLE, reps: int = 5, only=None, log=print) -> dict:
             "best": best["config"], "ms": best["ms"], "tflops": best["tflops"],
             "log2_terms": mean_log2_terms(rows),
             "units_per_cu": best["config"].get("total_units", 0) / cus,
-            "all": [{"config": r["config"], "ms": r["ms"]} for r in recs],
+            "all": [{"config": r["config"], "ms": r["ms"], "ms_passes": r["ms_passes"],
+                     "units": r["units"]} for r in recs],
         }
     os.makedirs(os.path.dirname(out), exist_ok=True)
     with open(out, "w") as f:

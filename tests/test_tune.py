@@ -43,3 +43,21 @@ def test_units_scale_with_the_device_and_other_devices_get_nothing():
     assert half["persistent"] == full["persistent"]
     assert tune.best_config(4096, 304, "gfx942", log2_terms=14.9) == {}
     assert tune.choose_class({"workloads": {}}, 1, 20.0) is None
+
+
+def test_tuned_inner_length_passes_through(tmp_path):
+    # a table whose batch class tuned an inner-sweep length returns it with
+    # the other settings; the committed table's entries carry only keys the
+    # batch constructor takes
+    import json
+
+    table = {"device": "gfx950", "cus": 256, "workloads": {
+        "batched_uneven": {"best": {"total_units": 5120, "persistent": False, "target_inner": 256},
+                           "log2_terms": 15.2}}}
+    path = tmp_path / "t.json"
+    path.write_text(json.dumps(table))
+    cfg = tune.best_config(4096, 128, ARCH, path=str(path), log2_terms=15.0)
+    assert cfg == {"total_units": 2560, "persistent": False, "target_inner": 256}
+    for name, rec in TABLE["workloads"].items():
+        assert set(rec["best"]) <= {"total_units", "persistent", "target_inner"}, name
+        assert set(rec["best"]) <= set(tune.GRIDS[name]), name
