@@ -314,14 +314,16 @@ def run_cpu_baseline(seconds: float):
 
 # ------------------------------------------------------------ GPU leg --
 
-def _traffic_from_profile():
-    """HBM bytes per evaluation of the batch, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel), or
-    None when that summary is absent."""
+def _traffic_from_profile(key: str = "traffic_bytes_per_launch"):
+    """HBM bytes per evaluation of the batch from the committed rocprofv3
+    summary, or None when it is absent.  traffic_bytes_per_launch: reads
+    counted by request size (TCC_EA0_RDREQ, _32B, _128B) plus WRITE_SIZE;
+    traffic_bytes_per_launch_copy_scaled: FETCH_SIZE x 2 + WRITE_SIZE x 1,
+    the factors a copy kernel's counts give."""
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch"):
+                if line.split("=")[0].strip() == key:
                     return float(line.split("=")[1].split()[0])
     except OSError:
         pass
@@ -565,8 +567,10 @@ def main():
     ap.add_argument("--replay", choices=["eager", "graph"], default="eager",
                     help="profiling runs: direct launches or hipGraph replays")
     ap.add_argument("--copy-calibration", type=int, default=None, metavar="MIB",
-                    help="profiling runs: one device-to-device copy of MIB MiB (torch), to "
-                         "calibrate FETCH_SIZE / WRITE_SIZE counters")
+                    help="profiling runs: one device-to-device copy of MIB MiB (the runtime's "
+                         "copy kernel) and one elementwise kernel reading and writing MIB MiB "
+                         "with ordinary cached loads and stores (torch add), to calibrate "
+                         "FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
 
     if args.cpu_child is not None:
@@ -581,7 +585,9 @@ def main():
         torch.cuda.synchronize()
         y.copy_(x)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes": 4 * n}))
+        torch.add(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": 4 * n, "add_bytes_read": 4 * n, "add_bytes_written": 4 * n}))
         return 0
     if args.profile is not None:
         import piquasso_amd as pa
@@ -861,6 +867,11 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": _traffic_from_profile(),
+            "traffic_copy_scaled": _traffic_from_profile("traffic_bytes_per_launch_copy_scaled"),
+            "traffic_method": "reads counted by request size (TCC_EA0_RDREQ, _32B, _128B: the sweep "
+                              "kernels issue 64-byte requests only) + WRITE_SIZE; FETCH_SIZE counts "
+                              "128-byte requests as 64 bytes on gfx950, so the copy-kernel factor 2 "
+                              "applies to 128-byte streams only (traffic_copy_scaled)",
             "kernel_ms": kernel_avg,
             "flops_per_launch": flops_exec,
             "flops_per_launch_as_given": flops_batch,
