@@ -82,3 +82,22 @@ def test_both_orientations_agree(gpu, workload):
     assert stats[True]["flops"] == given
     assert stats[False]["flops"] < 0.8 * given
     assert not np.array_equal(vals[True], vals[False])  # some items took the other sweep
+
+
+def test_timing_modes_leave_results_exact(workload):
+    """pqp_batch_time in every mode gives positive finite times, the
+    sweep-only time is at most the whole evaluation's, and the results of
+    the last timed evaluation are the oracle's values."""
+    mats, rows, cols, ref = workload
+    k = slice(0, 512)
+    with pa.PermanentBatch(mats[k], rows[k], cols[k]) as b:
+        times = {}
+        for mode in ("eager", "graph", "kernel"):
+            ms = b.time(mode, reps=6)
+            assert ms.shape == (6,) and np.all(np.isfinite(ms)) and np.all(ms > 0), mode
+            times[mode] = float(np.median(ms))
+            got = b.results()
+            rel = np.abs(got - ref[k]) / np.abs(ref[k])
+            assert rel.max() <= 1e-10, (mode, float(rel.max()))
+        assert times["kernel"] <= times["eager"] * 1.05, times
+        assert b.time("eager", reps=0).shape == (0,)
