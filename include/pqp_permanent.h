@@ -224,6 +224,14 @@ int pqp_batch_error_bounds(pqp_batch *b, double *bounds);
  * ms[k] = elapsed milliseconds of evaluation k. */
 int pqp_batch_time(pqp_batch *b, int mode, int reps, float *ms);
 
+/* Time `reps` >= 1 evaluations (mode PQP_RUN_GRAPH or PQP_RUN_EAGER) issued
+ * back to back between one pair of HIP events on the batch stream, after
+ * the stream has drained: *ms = elapsed milliseconds of all of them, every
+ * kernel of an evaluation included (ms / reps is the period of one).  A
+ * measurement entry with no counterpart in the reference (its benchmarks
+ * time host wall clock, benchmarks/permanent_jax_ffi_cuda.py:228-259). */
+int pqp_batch_time_span(pqp_batch *b, int mode, int reps, float *ms);
+
 /* Planning statistics: algorithmic FLOP (sum of T * (2 m_c + 6 (n-1) + 4)),
  * Gray terms, wave work units, device arena size, and lane terms (sum over
  * units of 64 lanes x inner states: terms / lane_terms is the fraction of

@@ -82,6 +82,7 @@ SIGNATURES = {
     "pqp_batch_results": (_c.c_int, [_vp, _dp, _dp]),
     "pqp_batch_error_bounds": (_c.c_int, [_vp, _dp]),
     "pqp_batch_time": (_c.c_int, [_vp, _c.c_int, _c.c_int, _fp]),
+    "pqp_batch_time_span": (_c.c_int, [_vp, _c.c_int, _c.c_int, _fp]),
     "pqp_batch_stats": (_c.c_int, [_vp, _dp, _i64p, _i64p, _i64p, _i64p]),
     "pqp_batch_destroy": (None, [_vp]),
     "pqp_batch_gradients": (_c.c_int, [_vp, _c.POINTER(_dp)]),

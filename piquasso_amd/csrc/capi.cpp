@@ -2073,6 +2073,37 @@ int pqp_batch_time(pqp_batch *b, int mode, int reps, float *ms) {
   return st;
 }
 
+int pqp_batch_time_span(pqp_batch *b, int mode, int reps, float *ms) {
+  CallRange range_("pqp_batch_time_span");
+  if (!b || reps < 1 || !ms) return fail(PQP_ERR_INVALID, "invalid timing arguments");
+  if (mode != PQP_RUN_GRAPH && mode != PQP_RUN_EAGER)
+    return fail(PQP_ERR_INVALID, "span timing takes the graph or eager mode");
+  // the graph is built outside the timed span
+  if (mode == PQP_RUN_GRAPH && !b->graph_exec) {
+    int st = pqp_batch_run(b, PQP_RUN_GRAPH);
+    if (st != PQP_OK) return st;
+  }
+  PQP_HIP(hipStreamSynchronize(b->x.stream));
+  hipEvent_t e0, e1;
+  PQP_HIP(hipEventCreate(&e0));
+  hipError_t he = hipEventCreate(&e1);
+  if (he != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(PQP_ERR_DEVICE, std::string("hipEventCreate: ") + hipGetErrorString(he));
+  }
+  int st = PQP_OK;
+  (void)hipEventRecord(e0, b->x.stream);
+  for (int r = 0; r < reps && st == PQP_OK; ++r) st = pqp_batch_run(b, mode);
+  (void)hipEventRecord(e1, b->x.stream);
+  he = hipEventSynchronize(e1);
+  if (he != hipSuccess && st == PQP_OK)
+    st = fail(PQP_ERR_DEVICE, std::string("hipEventSynchronize: ") + hipGetErrorString(he));
+  if (st == PQP_OK) (void)hipEventElapsedTime(ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return st;
+}
+
 int pqp_batch_stats(const pqp_batch *b, double *flops, int64_t *terms, int64_t *units,
                     int64_t *arena_bytes, int64_t *lane_terms) {
   if (!b) return fail(PQP_ERR_INVALID, "batch is null");

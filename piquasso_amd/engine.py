@@ -279,6 +279,18 @@ class PermanentBatch:
         )
         return ms[:reps].astype(np.float64)
 
+    def time_span(self, mode: str = "graph", reps: int = 10) -> float:
+        """Milliseconds of `reps` evaluations issued back to back between
+        one pair of HIP events on the batch stream (after it has drained),
+        every kernel of each evaluation included; divide by reps for the
+        period of one evaluation.  mode "graph" or "eager"."""
+        if mode not in ("graph", "eager"):
+            raise ValueError("time_span takes mode 'graph' or 'eager'")
+        code = {"eager": _capi.PQP_RUN_EAGER, "graph": _capi.PQP_RUN_GRAPH}[mode]
+        ms = ctypes.c_float(0.0)
+        _capi.check(_capi.lib().pqp_batch_time_span(self._h, code, int(reps), ctypes.byref(ms)))
+        return float(ms.value)
+
     def stats(self) -> dict:
         flops = ctypes.c_double(0)
         terms = ctypes.c_int64(0)

@@ -101,3 +101,11 @@ def test_timing_modes_leave_results_exact(workload):
             assert rel.max() <= 1e-10, (mode, float(rel.max()))
         assert times["kernel"] <= times["eager"] * 1.05, times
         assert b.time("eager", reps=0).shape == (0,)
+        # one pair of events around back-to-back evaluations
+        for mode in ("graph", "eager"):
+            span = b.time_span(mode, 8)
+            assert np.isfinite(span) and span > 0, mode
+            rel = np.abs(b.results() - ref[k]) / np.abs(ref[k])
+            assert rel.max() <= 1e-10, (mode, float(rel.max()))
+        with pytest.raises(ValueError):
+            b.time_span("kernel", 4)
