@@ -330,7 +330,7 @@ def _traffic_from_profile(key: str = "traffic_bytes_per_launch"):
     return None
 
 
-def _gradient_leg(pa, W, reps: int, cpu=None):
+def _gradient_leg(pa, W, reps: int, cpu=None, steps: int = 20):
     """Gradients of 256 permanents, n ~ U{12..20}, unit multiplicities, Haar
     48-mode submatrices: one Gray sweep per permanent gives the permanent and
     all n^2 entries of its gradient, in one hipGraph.  Euler's identity
@@ -347,14 +347,18 @@ def _gradient_leg(pa, W, reps: int, cpu=None):
         # both launch modes of the same sweeps, timed alike after untimed
         # evaluations of their own (the clock rises under sustained load); the
         # faster is quoted
-        ms_by = {}
+        # (device time per evaluation: one pair of HIP events around `steps`
+        # back-to-back evaluations, as for the batched permanents; the
+        # evaluations timed one at a time are kept beside it)
+        ms_by, iso_by = {}, {}
         for m in ("graph", "eager"):
             for _ in range(max(3, reps)):
                 gb.run(m)
             gb.sync()
-            ms_by[m] = gb.time(m, reps=reps)
+            ms_by[m] = gb.time_span(m, steps) / steps
+            iso_by[m] = float(np.median(gb.time(m, reps=reps)))
         st = gb.stats()
-    launch = min(ms_by, key=lambda m: float(np.median(ms_by[m])))
+    launch = min(ms_by, key=ms_by.get)
     ms = ms_by[launch]
     resid = 0.0
     for a, r, p, g in zip(mats, rows, perms, grads):
@@ -377,10 +381,12 @@ def _gradient_leg(pa, W, reps: int, cpu=None):
         "fp64_peak_frac": st["flops"] / (med * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
         "flop_model": "sum over items of T * (8 m_c + 18 (n - 1) + 4), T = Gray terms",
         "launch": launch,
-        "ms_by_launch": {m: float(np.median(v)) for m, v in ms_by.items()},
+        "ms_by_launch": {m: float(v) for m, v in ms_by.items()},
+        "ms_isolated_by_launch": iso_by,
         "what": "256 gradients d perm/dA (n~U{12..20}, 48-mode Haar submatrices): one Gray sweep "
                 "per permanent for the permanent and all n^2 gradient entries; hipGraph replay and "
-                "direct launches timed alike, the faster quoted",
+                "direct launches timed alike (HIP events around back-to-back evaluations), the "
+                "faster quoted",
     }
 
 
@@ -774,7 +780,7 @@ def main():
                 "packed FP32, which this build does not emit",
     }
 
-    grad = _gradient_leg(pa, W, reps=max(3, min(args.steps, 10)), cpu=cpu)
+    grad = _gradient_leg(pa, W, reps=max(3, min(args.steps, 10)), cpu=cpu, steps=max(args.steps, 1))
     sampling = _sampling_leg(pa, W, cpu)
     tensor = _tensor_leg(pa, W, cpu)
     probabilities = _probabilities_leg(pa, W, cpu)
